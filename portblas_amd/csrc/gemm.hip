@@ -1,0 +1,307 @@
+// FP32 / FP64 GEMM entry points: _gemm, _gemm_batched, _gemm_strided_batched.
+//
+// Front end (validation, alpha == 0, default strides) restates
+// src/interface/gemm_interface.hpp:105-240.  The tile kernels are the MFMA
+// engine of gemm_core.hpp (the "plain" FP32/FP64 method) and an FP32 vector
+// (SIMT) kernel used as the FP32-SIMT variant of pb_gemm_ex.
+#include "gemm_launch.hpp"
+#include "gemm_front.hpp"
+
+namespace pbgemm {
+
+using F128 = TileShape<128, 128, 32, 64, 64>;
+using F64 = TileShape<64, 64, 32, 32, 32>;
+using D128 = TileShape<128, 128, 16, 64, 64>;
+using D64 = TileShape<64, 64, 16, 32, 32>;
+
+// ---------------------------------------------------------------------------
+// alpha == 0: C <- beta * C exactly as gemm_interface.hpp:112-139 does it.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void scal_vector_kernel(int64_t n, T beta, T* c) {
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < n;
+       i += int64_t(gridDim.x) * blockDim.x)
+    c[i] = (beta == T(0)) ? T(0) : beta * c[i];  // _scal: alpha == 0 assigns zero
+}
+
+template <typename T>
+__global__ void scal_matrix_kernel(int64_t rows, int64_t cols, T beta, T* c, int64_t ldc) {
+  const int64_t total = rows * cols;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = i % rows, col = i / rows;
+    c[r + col * ldc] *= beta;  // _scal_matrix: multiply (NaN * 0 stays NaN)
+  }
+}
+
+template <typename T>
+pb_status_t scal_matrix(pb_handle_t h, int64_t rows, int64_t cols, T beta, T* c, int64_t ldc) {
+  if (beta == T(1) || rows <= 0 || cols <= 0) return PB_STATUS_SUCCESS;  // :505-506
+  const int64_t total = rows * cols;
+  int64_t g = pb::ceil_div(total, 256);
+  if (g > 8 * h->num_cus) g = 8 * h->num_cus;
+  hipLaunchKernelGGL(scal_matrix_kernel<T>, dim3(unsigned(g)), dim3(256), 0, h->stream, rows,
+                     cols, beta, c, ldc);
+  PB_CHECK_LAUNCH();
+  return PB_STATUS_SUCCESS;
+}
+
+template <typename T>
+pb_status_t alpha_zero(pb_handle_t h, int64_t M, int64_t N, T beta, T* C, int64_t ldc,
+                       int64_t stridec, int64_t batch, bool interleaved) {
+  if (M <= 0 || N <= 0 || batch <= 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(C != nullptr, "null C");
+  if (interleaved) {
+    // Rows r*batch + b are contiguous: one (M*batch) x N matrix of pitch ldc*batch.
+    return scal_matrix<T>(h, M * batch, N, beta, C, ldc * batch);
+  }
+  const int64_t size_c = ldc * N;
+  if (size_c == stridec) {
+    if (ldc == M) {
+      const int64_t n = size_c * batch;
+      int64_t g = pb::ceil_div(n, 256);
+      if (g > 8 * h->num_cus) g = 8 * h->num_cus;
+      hipLaunchKernelGGL(scal_vector_kernel<T>, dim3(unsigned(g)), dim3(256), 0, h->stream, n,
+                         beta, C);
+      PB_CHECK_LAUNCH();
+      return PB_STATUS_SUCCESS;
+    }
+    return scal_matrix<T>(h, M, N * batch, beta, C, ldc);
+  }
+  for (int64_t b = 0; b < batch; ++b) PB_TRY(scal_matrix<T>(h, M, N, beta, C + b * stridec, ldc));
+  return PB_STATUS_SUCCESS;
+}
+
+// ---------------------------------------------------------------------------
+// FP32 SIMT kernel: 128 x 128 tile, 16 x 16 threads each owning an 8 x 8
+// register tile (rows ty + 16 i, cols tx + 16 j), sequential FMA over k.
+// ---------------------------------------------------------------------------
+template <bool KCA, bool KCB, bool EDGE>
+__global__ __launch_bounds__(256) void gemm_simt_kernel(GemmProblem<float> p) {
+  using TL = Tile<float, F128, KCA, KCB>;
+  __shared__ __attribute__((aligned(16))) float lds[2 * TL::STAGE_ELEMS];
+  int64_t tm, tn;
+  tile_of(p, blockIdx.x, tm, tn);
+  const int64_t m0 = tm * 128, n0 = tn * 128;
+  const int tx = threadIdx.x % 16, ty = threadIdx.x / 16;
+  for (int64_t b = blockIdx.y; b < p.batch; b += gridDim.y) {
+    float acc[8][8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) acc[i][j] = 0.f;
+    Operand<float> A{p.a[0] + b * p.a_stride, p.a_rs, p.a_ks, p.M};
+    Operand<float> B{p.b[0] + b * p.b_stride, p.b_rs, p.b_ks, p.N};
+    TL::template mainloop_fn<EDGE>(A, B, m0, n0, 0, p.K, lds, [&](const float* st) {
+      const float* As = st;
+      const float* Bs = st + TL::A_ELEMS;
+#pragma unroll 2
+      for (int kk = 0; kk < 32; kk += 4) {
+        float4 a[8], bb[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+          a[i] = *reinterpret_cast<const float4*>(As + (ty + 16 * i) * TL::LDK + kk);
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          bb[j] = *reinterpret_cast<const float4*>(Bs + (tx + 16 * j) * TL::LDK + kk);
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            acc[i][j] = fmaf(a[i].x, bb[j].x, acc[i][j]);
+            acc[i][j] = fmaf(a[i].y, bb[j].y, acc[i][j]);
+            acc[i][j] = fmaf(a[i].z, bb[j].z, acc[i][j]);
+            acc[i][j] = fmaf(a[i].w, bb[j].w, acc[i][j]);
+          }
+      }
+    });
+    float* c = p.c + b * p.c_stride;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int64_t m = m0 + ty + 16 * i;
+      if (m >= p.M) continue;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) {
+        const int64_t n = n0 + tx + 16 * j;
+        if (n >= p.N) continue;
+        float* cp = c + m * p.c_rs + n * p.ldc;
+        const float v = p.alpha * acc[i][j];
+        *cp = (p.beta != 0.f) ? v + p.beta * *cp : v;
+      }
+    }
+  }
+}
+
+template <bool EDGE>
+void dispatch_simt(pb_handle_t h, const GemmProblem<float>& p, dim3 grid, bool kca, bool kcb) {
+#define PB_LAUNCH(KA, KB)                                                                 \
+  hipLaunchKernelGGL((gemm_simt_kernel<KA, KB, EDGE>), grid, dim3(256), 0, h->stream, p)
+  if (kca && kcb) PB_LAUNCH(true, true);
+  else if (kca) PB_LAUNCH(true, false);
+  else if (kcb) PB_LAUNCH(false, true);
+  else PB_LAUNCH(false, false);
+#undef PB_LAUNCH
+}
+
+pb_status_t launch_simt(pb_handle_t h, GemmProblem<float> p, bool kca, bool kcb, bool vec) {
+  p.tiles_m = pb::ceil_div(p.M, 128);
+  p.tiles_n = pb::ceil_div(p.N, 128);
+  p.full_m = p.M / 128;
+  p.full_n = p.N / 128;
+  p.splits = 1;
+  const int64_t tiles = p.tiles_m * p.tiles_n;
+  PB_REQUIRE(tiles <= 0x7fffffffLL, "problem too large");
+  const unsigned gy = unsigned(p.batch < 65535 ? p.batch : 65535);
+  const int64_t interior = vec ? p.full_m * p.full_n : 0;
+  if (interior > 0) {
+    p.tile_set = kInteriorTiles;
+    dispatch_simt<false>(h, p, dim3{unsigned(interior), gy}, kca, kcb);
+    PB_CHECK_LAUNCH();
+  }
+  if (tiles - interior > 0) {
+    p.tile_set = vec ? kEdgeTiles : kAllTiles;
+    dispatch_simt<true>(h, p, dim3{unsigned(tiles - interior), gy}, kca, kcb);
+    PB_CHECK_LAUNCH();
+  }
+  return PB_STATUS_SUCCESS;
+}
+
+// Tile shape and split count for the plain MFMA path: big tiles when there
+// are enough of them to fill the chip, else 64 x 64 tiles, else split K.
+struct Plan { bool big; int splits; };
+
+inline Plan plan_plain(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch, int bk) {
+  const int64_t cus = h->num_cus;
+  const int64_t t128 = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
+  const int64_t t64 = pb::ceil_div(M, 64) * pb::ceil_div(N, 64) * batch;
+  if (t128 >= 2 * cus) return {true, 1};
+  if (t64 >= 2 * cus) return {false, 1};
+  int64_t s = pb::ceil_div(2 * cus, t64);
+  const int64_t smax = K / (4 * bk);
+  if (s > smax) s = smax;
+  if (s < 1) s = 1;
+  return {false, int(s)};
+}
+
+template <typename T>
+pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt) {
+  GemmProblem<T> p{};
+  p.M = g.M; p.N = g.N; p.K = g.K;
+  p.alpha = g.alpha; p.beta = g.beta;
+  p.nterms = 1;
+  p.a[0] = g.a;
+  p.b[0] = g.b;
+  p.c = g.c;
+  p.batch = g.batch;
+  const bool vec = set_operands(p, g.ta, g.tb, g.lda, g.sa, g.ldb, g.sb, g.interleaved);
+  p.c_rs = g.interleaved ? g.batch : 1;
+  p.ldc = g.interleaved ? g.ldc * g.batch : g.ldc;
+  p.c_stride = g.interleaved ? 1 : g.sc;
+  p.c_vec = !g.interleaved && aligned16(g.c) && g.ldc % 4 == 0 &&
+            (g.batch == 1 || g.sc % 4 == 0);
+  const bool kca = g.ta == 't', kcb = g.tb == 'n';
+  if constexpr (sizeof(T) == 4) {
+    if (simt) return launch_simt(h, p, kca, kcb, vec);
+    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 32);
+    if (pl.big) return launch_gemm<T, F128>(h, p, kca, kcb, vec, pl.splits);
+    return launch_gemm<T, F64>(h, p, kca, kcb, vec, pl.splits);
+  } else {
+    (void)simt;
+    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 16);
+    if (pl.big) return launch_gemm<T, D128>(h, p, kca, kcb, vec, pl.splits);
+    return launch_gemm<T, D64>(h, p, kca, kcb, vec, pl.splits);
+  }
+}
+
+template <typename T>
+pb_status_t gemm_entry(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64_t K, T alpha,
+                       const T* a, int64_t lda, int64_t sa, const T* b, int64_t ldb, int64_t sb,
+                       T beta, T* c, int64_t ldc, int64_t sc, int64_t batch,
+                       pb_batch_type_t bt, bool simt) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  GemmCall<T> g;
+  bool done = false;
+  PB_TRY(gemm_front<T>(h, ta, tb, M, N, K, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                       batch, bt, g, done));
+  if (done) return PB_STATUS_SUCCESS;
+  if (g.alpha == T(0))
+    return alpha_zero<T>(h, g.M, g.N, g.beta, g.c, g.ldc, g.sc, g.batch, g.interleaved);
+  return run_plain<T>(h, g, simt);
+}
+
+}  // namespace pbgemm
+
+// Used by pb_gemm_ex (gemm_lowp.hip) for the FP32 arithmetics.
+namespace pbgemm {
+pb_status_t sgemm_plain(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                        float alpha, const float* a, int64_t lda, int64_t sa, const float* b,
+                        int64_t ldb, int64_t sb, float beta, float* c, int64_t ldc, int64_t sc,
+                        int64_t batch, bool simt) {
+  return gemm_entry<float>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                           batch, PB_BATCH_STRIDED, simt);
+}
+pb_status_t sgemm_alpha_zero(pb_handle_t h, int64_t M, int64_t N, float beta, float* C,
+                             int64_t ldc, int64_t stridec, int64_t batch) {
+  return alpha_zero<float>(h, M, N, beta, C, ldc, stridec, batch, false);
+}
+}  // namespace pbgemm
+
+using namespace pbgemm;
+
+extern "C" {
+
+pb_status_t pb_sgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                     float alpha, const float* a, int64_t lda, const float* b, int64_t ldb,
+                     float beta, float* c, int64_t ldc) {
+  // internal::_gemm: batch 1, strides 0 (gemm_interface.hpp:189-198).
+  return gemm_entry<float>(h, ta, tb, m, n, k, alpha, a, lda, 0, b, ldb, 0, beta, c, ldc, 0, 1,
+                           PB_BATCH_STRIDED, false);
+}
+
+pb_status_t pb_dgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                     double alpha, const double* a, int64_t lda, const double* b, int64_t ldb,
+                     double beta, double* c, int64_t ldc) {
+  return gemm_entry<double>(h, ta, tb, m, n, k, alpha, a, lda, 0, b, ldb, 0, beta, c, ldc, 0, 1,
+                            PB_BATCH_STRIDED, false);
+}
+
+pb_status_t pb_sgemm_batched(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                             float alpha, const float* a, int64_t lda, const float* b,
+                             int64_t ldb, float beta, float* c, int64_t ldc, int64_t batch,
+                             pb_batch_type_t bt) {
+  int64_t sa = 0, sb = 0, sc = 0;
+  default_strides(ta, tb, m, n, k, lda, ldb, ldc, bt, sa, sb, sc);
+  return gemm_entry<float>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                           batch, bt, false);
+}
+
+pb_status_t pb_dgemm_batched(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                             double alpha, const double* a, int64_t lda, const double* b,
+                             int64_t ldb, double beta, double* c, int64_t ldc, int64_t batch,
+                             pb_batch_type_t bt) {
+  int64_t sa = 0, sb = 0, sc = 0;
+  default_strides(ta, tb, m, n, k, lda, ldb, ldc, bt, sa, sb, sc);
+  return gemm_entry<double>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                            batch, bt, false);
+}
+
+pb_status_t pb_sgemm_strided_batched(pb_handle_t h, char ta, char tb, int64_t m, int64_t n,
+                                     int64_t k, float alpha, const float* a, int64_t lda,
+                                     int64_t sa, const float* b, int64_t ldb, int64_t sb,
+                                     float beta, float* c, int64_t ldc, int64_t sc,
+                                     int64_t batch) {
+  return gemm_entry<float>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                           batch, PB_BATCH_STRIDED, false);
+}
+
+pb_status_t pb_dgemm_strided_batched(pb_handle_t h, char ta, char tb, int64_t m, int64_t n,
+                                     int64_t k, double alpha, const double* a, int64_t lda,
+                                     int64_t sa, const double* b, int64_t ldb, int64_t sb,
+                                     double beta, double* c, int64_t ldc, int64_t sc,
+                                     int64_t batch) {
+  return gemm_entry<double>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                            batch, PB_BATCH_STRIDED, false);
+}
+
+}  // extern "C"

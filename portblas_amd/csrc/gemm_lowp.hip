@@ -1,0 +1,425 @@
+// Reduced-precision GEMMs with FP32 accumulation: BF16x3 (split precision),
+// BF16 and per-tensor-scaled FP8-e4m3, plus the BF16-input GEMM and the tile
+// configuration table the autotuner searches.
+//
+// FP32 inputs go through one pre-pass per operand that reads op(X) once and
+// writes it K-contiguous (transposing through LDS when X is M/N-contiguous) in
+// the low-precision format:
+//   BF16x3:  hi = bf16(x), lo = bf16(x - hi); C = A_hi B_lo + A_lo B_hi + A_hi B_hi
+//   BF16:    hi only
+//   FP8:     q = e4m3(x * 448 / amax(X)); the epilogue multiplies by
+//            amax(A) amax(B) / 448^2 (read on the device, no host round trip)
+// The GEMM then runs the MFMA tile engine on the packed operands (every
+// operand K-contiguous and 16-byte aligned, so interior tiles take the
+// unguarded fast path).
+#include "gemm_front.hpp"
+#include "gemm_launch.hpp"
+#include "lowp.hpp"
+
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace pbgemm {
+
+pb_status_t sgemm_plain(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                        float alpha, const float* a, int64_t lda, int64_t sa, const float* b,
+                        int64_t ldb, int64_t sb, float beta, float* c, int64_t ldc, int64_t sc,
+                        int64_t batch, bool simt);
+pb_status_t sgemm_alpha_zero(pb_handle_t h, int64_t M, int64_t N, float beta, float* C,
+                             int64_t ldc, int64_t stridec, int64_t batch);
+
+// ---------------------------------------------------------------------------
+// Pre-pass kernels
+// ---------------------------------------------------------------------------
+// amax over the logical rows x K elements of op(X) (all batches); |x| bits are
+// monotone for non-negative floats, so an integer max is exact.  NaN skipped.
+__global__ void amax_kernel(const float* __restrict__ x, int64_t rs, int64_t ks, int64_t rows,
+                            int64_t K, int64_t stride, int64_t batch, unsigned int* out) {
+  const int64_t per = rows * K;
+  const int64_t total = per * batch;
+  unsigned int best = 0;
+  for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
+       i += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t b = i / per, e = i % per;
+    const int64_t r = rs == 1 ? e % rows : e / K;
+    const int64_t k = rs == 1 ? e / rows : e % K;
+    const float v = fabsf(x[b * stride + r * rs + k * ks]);
+    if (v == v) {
+      unsigned int u;
+      __builtin_memcpy(&u, &v, 4);
+      best = u > best ? u : best;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned int o = __shfl_xor(best, off, 64);
+    best = o > best ? o : best;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+}
+
+// scales[0] = 448 / amax_a, scales[1] = 448 / amax_b, scales[2] = the
+// epilogue factor amax_a * amax_b / 448^2 (1 for an all-zero operand).
+__global__ void fp8_scales_kernel(const unsigned int* amax_bits, float* scales) {
+  float a, b;
+  __builtin_memcpy(&a, &amax_bits[0], 4);
+  __builtin_memcpy(&b, &amax_bits[1], 4);
+  const float sa = (a > 0.f && a < 3.0e38f) ? 448.0f / a : 1.0f;
+  const float sb = (b > 0.f && b < 3.0e38f) ? 448.0f / b : 1.0f;
+  scales[0] = sa;
+  scales[1] = sb;
+  scales[2] = (1.0f / sa) * (1.0f / sb);
+}
+
+__global__ void fill_zero_u32(unsigned int* p, int n) {
+  if (int(threadIdx.x) < n) p[threadIdx.x] = 0u;
+}
+
+enum PackMode { kPackBf16 = 0, kPackBf16Split = 1, kPackFp8 = 2 };
+
+// Writes op(X) (rows x K, element (r, k) at x[r*rs + k*ks], batch b at
+// + b*stride) as dst[b][r][k] with row pitch kp (zero for k in [K, kp)).
+// 64 x 64 tiles through LDS: reads walk X's contiguous dimension, writes walk K.
+template <int MODE>
+__global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, int64_t rs,
+                                                   int64_t ks, int64_t rows, int64_t K,
+                                                   int64_t stride, int64_t kp, void* dst_hi,
+                                                   void* dst_lo, const float* scales,
+                                                   int scale_idx) {
+  __shared__ float tile[64][65];
+  const int64_t r0 = int64_t(blockIdx.x) * 64;
+  const int64_t k0 = int64_t(blockIdx.y) * 64;
+  const int64_t b = blockIdx.z;
+  const float* xb = x + b * stride;
+  const bool rows_contig = (rs == 1);
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int rr = rows_contig ? e % 64 : e / 64;
+    const int kk = rows_contig ? e / 64 : e % 64;
+    const int64_t r = r0 + rr, k = k0 + kk;
+    tile[rr][kk] = (r < rows && k < K) ? xb[r * rs + k * ks] : 0.f;
+  }
+  __syncthreads();
+  const float s = (MODE == kPackFp8) ? scales[scale_idx] : 1.0f;
+  for (int e = threadIdx.x; e < 64 * 64; e += 256) {
+    const int rr = e / 64, kk = e % 64;
+    const int64_t r = r0 + rr, k = k0 + kk;
+    if (r >= rows || k >= kp) continue;
+    const int64_t o = (b * rows + r) * kp + k;
+    const float v = tile[rr][kk];
+    if constexpr (MODE == kPackFp8) {
+      static_cast<uint8_t*>(dst_hi)[o] = pblowp::f32_to_e4m3(v * s);
+    } else {
+      const uint16_t hi = pblowp::f32_to_bf16(v);
+      static_cast<uint16_t*>(dst_hi)[o] = hi;
+      if constexpr (MODE == kPackBf16Split)
+        static_cast<uint16_t*>(dst_lo)[o] = pblowp::f32_to_bf16(v - pblowp::bf16_to_f32(hi));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Tile configurations (the autotuner's search space for the BF16 engine):
+// 8 tile shapes x 5 split-K factors.
+// ---------------------------------------------------------------------------
+using B128 = TileShape<128, 128, 64, 64, 64>;
+using B128x64 = TileShape<128, 64, 64, 64, 32>;
+using B64x128 = TileShape<64, 128, 64, 32, 64>;
+using B64 = TileShape<64, 64, 64, 32, 32>;
+using B256x128 = TileShape<256, 128, 64, 128, 64>;
+using B128x256 = TileShape<128, 256, 64, 64, 128>;
+using B128k32 = TileShape<128, 128, 32, 64, 64>;
+using B256 = TileShape<256, 256, 64, 128, 128>;
+using Q128 = TileShape<128, 128, 128, 64, 64>;  // FP8 (K group of 128)
+
+constexpr int kNumShapes = 8;
+constexpr int kSplits[] = {1, 2, 4, 8, 16};
+constexpr int kNumSplits = 5;
+constexpr int kNumConfigs = kNumShapes * kNumSplits;
+const char* kShapeNames[kNumShapes] = {"128x128x64", "128x64x64", "64x128x64", "64x64x64",
+                                      "256x128x64", "128x256x64", "128x128x32", "256x256x64"};
+const int kShapeBM[kNumShapes] = {128, 128, 64, 64, 256, 128, 128, 256};
+const int kShapeBN[kNumShapes] = {128, 64, 128, 64, 128, 256, 128, 256};
+
+std::string& config_name_storage(int id) {
+  static std::vector<std::string> names = [] {
+    std::vector<std::string> v;
+    for (int s = 0; s < kNumShapes; ++s)
+      for (int k = 0; k < kNumSplits; ++k)
+        v.push_back(std::string(kShapeNames[s]) + "/splitk" + std::to_string(kSplits[k]));
+    return v;
+  }();
+  return names[id];
+}
+
+template <typename T>
+pb_status_t launch_config(pb_handle_t h, int id, const GemmProblem<T>& p, bool kca, bool kcb,
+                          bool vec) {
+  const int shape = id / kNumSplits, splits = kSplits[id % kNumSplits];
+  switch (shape) {
+    case 0: return launch_gemm<T, B128>(h, p, kca, kcb, vec, splits);
+    case 1: return launch_gemm<T, B128x64>(h, p, kca, kcb, vec, splits);
+    case 2: return launch_gemm<T, B64x128>(h, p, kca, kcb, vec, splits);
+    case 3: return launch_gemm<T, B64>(h, p, kca, kcb, vec, splits);
+    case 4: return launch_gemm<T, B256x128>(h, p, kca, kcb, vec, splits);
+    case 5: return launch_gemm<T, B128x256>(h, p, kca, kcb, vec, splits);
+    case 6: return launch_gemm<T, B128k32>(h, p, kca, kcb, vec, splits);
+    case 7: return launch_gemm<T, B256>(h, p, kca, kcb, vec, splits);
+  }
+  return pb::set_error(PB_STATUS_INVALID_VALUE, "config id %d out of range", id);
+}
+
+// Built-in rule (used when no learned selector is installed): the largest
+// tile that still gives every CU two workgroups, else split K.
+int default_config(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int64_t cus = h->num_cus;
+  auto tiles = [&](int bm, int bn) { return pb::ceil_div(M, bm) * pb::ceil_div(N, bn) * batch; };
+  if (tiles(256, 256) >= cus && M >= 2048 && N >= 2048) return 7 * kNumSplits;
+  if (tiles(128, 128) >= 2 * cus) return 0;
+  if (tiles(64, 64) >= 2 * cus) return 3 * kNumSplits;
+  const int64_t t = tiles(64, 64);
+  int s = 0;
+  while (s + 1 < kNumSplits && t * kSplits[s] < 2 * cus && K / kSplits[s + 1] >= 256) ++s;
+  return 3 * kNumSplits + s;
+}
+
+int selector_choose(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64_t K,
+                    int64_t batch);
+
+// ---------------------------------------------------------------------------
+// Runners
+// ---------------------------------------------------------------------------
+struct Scratch {
+  uint8_t* base = nullptr;
+  size_t off = 0;
+  template <typename U>
+  U* take(size_t count) {
+    U* p = reinterpret_cast<U*>(base + off);
+    off = pb::align_up(off + count * sizeof(U), 256);
+    return p;
+  }
+};
+
+// Runs C = alpha * sum_t op(A_t) op(B_t) + beta * C on BF16 (or FP8)
+// operands with config `id` (-1: pick one).
+template <typename T>
+pb_status_t run_lowp(pb_handle_t h, int id, GemmProblem<T> p, bool kca, bool kcb, bool vec) {
+  if (id < 0) {
+    id = selector_choose(h, 'n', 'n', p.M, p.N, p.K, p.batch);
+    if (id < 0) id = default_config(h, p.M, p.N, p.K, p.batch);
+  }
+  PB_REQUIRE(id >= 0 && id < kNumConfigs, "config id %d out of range", id);
+  return launch_config<T>(h, id, p, kca, kcb, vec);
+}
+
+pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall<float>& g) {
+  const int64_t M = g.M, N = g.N, K = g.K, batch = g.batch;
+  const bool fp8 = prec == PB_GEMM_FP8_E4M3;
+  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1), fp8 ? 16 : 8);
+  const size_t esz = fp8 ? 1 : 2;
+  const int nparts = prec == PB_GEMM_BF16X3 ? 2 : 1;
+  const size_t abytes = pb::align_up(size_t(M) * kp * esz * batch, 256);
+  const size_t bbytes = pb::align_up(size_t(N) * kp * esz * batch, 256);
+  void* ws = nullptr;
+  PB_TRY(pb::workspace(h, nparts * (abytes + bbytes) + 1024, &ws));
+  Scratch sc{static_cast<uint8_t*>(ws)};
+  void* a_hi = sc.take<uint8_t>(abytes);
+  void* a_lo = nparts == 2 ? sc.take<uint8_t>(abytes) : nullptr;
+  void* b_hi = sc.take<uint8_t>(bbytes);
+  void* b_lo = nparts == 2 ? sc.take<uint8_t>(bbytes) : nullptr;
+  float* scales = sc.take<float>(4);
+  unsigned int* amax_bits = sc.take<unsigned int>(2);
+
+  // Operand strides of op(A) (rows M) and op(B) (rows N), as for the engine.
+  const int64_t a_rs = g.ta == 'n' ? 1 : g.lda, a_ks = g.ta == 'n' ? g.lda : 1;
+  const int64_t b_rs = g.tb == 'n' ? g.ldb : 1, b_ks = g.tb == 'n' ? 1 : g.ldb;
+  if (fp8) {
+    // amax of each operand over all batches, then the scales (device side).
+    hipLaunchKernelGGL(fill_zero_u32, dim3(1), dim3(64), 0, h->stream, amax_bits, 2);
+    const int64_t ga = std::min<int64_t>(pb::ceil_div(M * K * batch, 256), 4 * h->num_cus);
+    const int64_t gb = std::min<int64_t>(pb::ceil_div(N * K * batch, 256), 4 * h->num_cus);
+    hipLaunchKernelGGL(amax_kernel, dim3(unsigned(std::max<int64_t>(ga, 1))), dim3(256), 0,
+                       h->stream, g.a, a_rs, a_ks, M, K, g.sa, batch, amax_bits);
+    hipLaunchKernelGGL(amax_kernel, dim3(unsigned(std::max<int64_t>(gb, 1))), dim3(256), 0,
+                       h->stream, g.b, b_rs, b_ks, N, K, g.sb, batch, amax_bits + 1);
+    hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(1), 0, h->stream, amax_bits, scales);
+    PB_CHECK_LAUNCH();
+  }
+  PB_REQUIRE(batch <= 65535, "batch too large for the pre-pass");
+  auto pack = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, void* hi,
+                  void* lo, int sidx) {
+    dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(kp, 64)), unsigned(batch)};
+    if (prec == PB_GEMM_BF16X3)
+      hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
+                         rows, K, stride, kp, hi, lo, scales, sidx);
+    else if (prec == PB_GEMM_BF16)
+      hipLaunchKernelGGL(pack_kernel<kPackBf16>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
+                         stride, kp, hi, lo, scales, sidx);
+    else
+      hipLaunchKernelGGL(pack_kernel<kPackFp8>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
+                         stride, kp, hi, lo, scales, sidx);
+  };
+  pack(g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0);
+  pack(g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1);
+  PB_CHECK_LAUNCH();
+
+  const bool c_vec = aligned16(g.c) && g.ldc % 4 == 0 && (batch == 1 || g.sc % 4 == 0);
+  if (fp8) {
+    GemmProblem<fp8_t> p{};
+    p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;
+    p.alpha_dev = scales + 2;
+    p.nterms = 1;
+    p.a[0] = static_cast<const uint8_t*>(a_hi);
+    p.b[0] = static_cast<const uint8_t*>(b_hi);
+    p.a_rs = kp; p.a_ks = 1; p.a_stride = M * kp;
+    p.b_rs = kp; p.b_ks = 1; p.b_stride = N * kp;
+    p.c = g.c; p.c_rs = 1; p.ldc = g.ldc; p.c_stride = g.sc; p.batch = batch; p.c_vec = c_vec;
+    const int64_t t = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
+    int splits = 1;
+    while (t * splits < 2 * h->num_cus && K / (splits * 2) >= 512 && splits < 16) splits *= 2;
+    return launch_gemm<fp8_t, Q128>(h, p, true, true, true, splits);
+  }
+  GemmProblem<bf16_t> p{};
+  p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;
+  const auto* ah = static_cast<const uint16_t*>(a_hi);
+  const auto* al = static_cast<const uint16_t*>(a_lo);
+  const auto* bh = static_cast<const uint16_t*>(b_hi);
+  const auto* bl = static_cast<const uint16_t*>(b_lo);
+  if (prec == PB_GEMM_BF16X3) {
+    // Small terms first, then the leading term.
+    p.nterms = 3;
+    p.a[0] = ah; p.b[0] = bl;
+    p.a[1] = al; p.b[1] = bh;
+    p.a[2] = ah; p.b[2] = bh;
+  } else {
+    p.nterms = 1;
+    p.a[0] = ah; p.b[0] = bh;
+  }
+  p.a_rs = kp; p.a_ks = 1; p.a_stride = M * kp;
+  p.b_rs = kp; p.b_ks = 1; p.b_stride = N * kp;
+  p.c = g.c; p.c_rs = 1; p.ldc = g.ldc; p.c_stride = g.sc; p.batch = batch; p.c_vec = c_vec;
+  return run_lowp<bf16_t>(h, -1, p, true, true, true);
+}
+
+// BF16-input GEMM on the caller's layout (no pre-pass).
+pb_status_t gemm_bf16_direct(pb_handle_t h, int id, const GemmCall<uint16_t>& g, float alpha,
+                             float beta, float* c) {
+  GemmProblem<bf16_t> p{};
+  p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = alpha; p.beta = beta;
+  p.nterms = 1;
+  p.a[0] = g.a;
+  p.b[0] = g.b;
+  p.batch = g.batch;
+  const bool vec = set_operands(p, g.ta, g.tb, g.lda, g.sa, g.ldb, g.sb, false);
+  p.c = c; p.c_rs = 1; p.ldc = g.ldc; p.c_stride = g.sc;
+  p.c_vec = aligned16(c) && g.ldc % 4 == 0 && (g.batch == 1 || g.sc % 4 == 0);
+  if (id < 0) {
+    id = selector_choose(h, g.ta, g.tb, g.M, g.N, g.K, g.batch);
+    if (id < 0) id = default_config(h, g.M, g.N, g.K, g.batch);
+  }
+  PB_REQUIRE(id >= 0 && id < kNumConfigs, "config id %d out of range", id);
+  return launch_config<bf16_t>(h, id, p, g.ta == 't', g.tb == 'n', vec);
+}
+
+pb_status_t gemm_bf16_entry(pb_handle_t h, int id, char ta, char tb, int64_t m, int64_t n,
+                            int64_t k, float alpha, const uint16_t* a, int64_t lda, int64_t sa,
+                            const uint16_t* b, int64_t ldb, int64_t sb, float beta, float* c,
+                            int64_t ldc, int64_t sc, int64_t batch) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  GemmCall<uint16_t> g;
+  bool done = false;
+  PB_TRY((gemm_front<float, uint16_t>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c,
+                                      ldc, sc, batch, PB_BATCH_STRIDED, g, done)));
+  if (done) return PB_STATUS_SUCCESS;
+  if (alpha == 0.f) return sgemm_alpha_zero(h, m, n, beta, c, ldc, sc, batch);
+  return gemm_bf16_direct(h, id, g, alpha, beta, c);
+}
+
+}  // namespace pbgemm
+
+using namespace pbgemm;
+
+extern "C" {
+
+pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t prec, char ta, char tb, int64_t m,
+                       int64_t n, int64_t k, float alpha, const float* a, int64_t lda,
+                       int64_t sa, const float* b, int64_t ldb, int64_t sb, float beta, float* c,
+                       int64_t ldc, int64_t sc, int64_t batch) {
+  if (prec == PB_GEMM_FP32_MFMA || prec == PB_GEMM_FP32_SIMT)
+    return sgemm_plain(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc, batch,
+                       prec == PB_GEMM_FP32_SIMT);
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  if (prec != PB_GEMM_BF16X3 && prec != PB_GEMM_BF16 && prec != PB_GEMM_FP8_E4M3)
+    return pb::set_error(PB_STATUS_INVALID_VALUE, "invalid precision %d", int(prec));
+  GemmCall<float> g;
+  bool done = false;
+  PB_TRY(gemm_front<float>(h, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                           batch, PB_BATCH_STRIDED, g, done));
+  if (done) return PB_STATUS_SUCCESS;
+  if (alpha == 0.f) return sgemm_alpha_zero(h, m, n, beta, c, ldc, sc, batch);
+  return gemm_ex_lowp(h, prec, g);
+}
+
+pb_status_t pb_gemm_bf16(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                         float alpha, const uint16_t* a, int64_t lda, int64_t sa,
+                         const uint16_t* b, int64_t ldb, int64_t sb, float beta, float* c,
+                         int64_t ldc, int64_t sc, int64_t batch) {
+  return gemm_bf16_entry(h, -1, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                         batch);
+}
+
+pb_status_t pb_gemm_bf16_config(pb_handle_t h, int id, char ta, char tb, int64_t m, int64_t n,
+                                int64_t k, float alpha, const uint16_t* a, int64_t lda,
+                                int64_t sa, const uint16_t* b, int64_t ldb, int64_t sb,
+                                float beta, float* c, int64_t ldc, int64_t sc, int64_t batch) {
+  if (id < 0 || id >= kNumConfigs)
+    return pb::set_error(PB_STATUS_INVALID_VALUE, "config id %d out of range", id);
+  return gemm_bf16_entry(h, id, ta, tb, m, n, k, alpha, a, lda, sa, b, ldb, sb, beta, c, ldc, sc,
+                         batch);
+}
+
+int pb_tune_num_configs(void) { return kNumConfigs; }
+
+const char* pb_tune_config_name(int id) {
+  if (id < 0 || id >= kNumConfigs) return "";
+  return config_name_storage(id).c_str();
+}
+
+pb_status_t pb_tune_time_config(pb_handle_t h, int id, char ta, char tb, int64_t m, int64_t n,
+                                int64_t k, const uint16_t* a, int64_t lda, const uint16_t* b,
+                                int64_t ldb, float* c, int64_t ldc, int64_t batch, int warmup,
+                                int reps, float* usec_out) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  PB_REQUIRE(usec_out != nullptr && reps > 0, "bad timing arguments");
+  PB_REQUIRE(id >= 0 && id < kNumConfigs, "config id %d out of range", id);
+  const char nta = pb::norm_trans(ta), ntb = pb::norm_trans(tb);
+  PB_REQUIRE(nta && ntb, "invalid transpose flag");
+  const int64_t sa = (nta == 'n' ? k : m) * lda, sb = (ntb == 'n' ? n : k) * ldb, sc = n * ldc;
+  for (int i = 0; i < warmup; ++i)
+    PB_TRY(gemm_bf16_entry(h, id, ta, tb, m, n, k, 1.f, a, lda, sa, b, ldb, sb, 0.f, c, ldc, sc,
+                           batch));
+  hipEvent_t e0, e1;
+  PB_CHECK_HIP(hipEventCreate(&e0));
+  PB_CHECK_HIP(hipEventCreate(&e1));
+  PB_CHECK_HIP(hipEventRecord(e0, h->stream));
+  pb_status_t st = PB_STATUS_SUCCESS;
+  for (int i = 0; i < reps && st == PB_STATUS_SUCCESS; ++i)
+    st = gemm_bf16_entry(h, id, ta, tb, m, n, k, 1.f, a, lda, sa, b, ldb, sb, 0.f, c, ldc, sc,
+                         batch);
+  hipError_t e = hipEventRecord(e1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != PB_STATUS_SUCCESS) return st;
+  if (e != hipSuccess)
+    return pb::set_error(PB_STATUS_DEVICE_ERROR, "timing failed: %s", hipGetErrorString(e));
+  *usec_out = 1000.f * ms / float(reps);
+  return PB_STATUS_SUCCESS;
+}
+
+}  // extern "C"

@@ -1,0 +1,129 @@
+// Host- and device-side internals shared by every translation unit of
+// libportblas_amd.so.  Not part of the public ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/portblas_amd.h"
+
+namespace pb {
+
+// ---------------------------------------------------------------------------
+// Error reporting: every C entry point returns a status and leaves a
+// per-thread message behind.
+// ---------------------------------------------------------------------------
+pb_status_t set_error(pb_status_t status, const char* fmt, ...);
+void clear_error();
+
+#define PB_CHECK_HIP(expr)                                                      \
+  do {                                                                          \
+    hipError_t pb_e_ = (expr);                                                  \
+    if (pb_e_ != hipSuccess)                                                    \
+      return ::pb::set_error(PB_STATUS_DEVICE_ERROR, "%s failed: %s (%s:%d)",   \
+                             #expr, hipGetErrorString(pb_e_), __FILE__,         \
+                             __LINE__);                                         \
+  } while (0)
+
+#define PB_CHECK_LAUNCH()                                                       \
+  do {                                                                          \
+    hipError_t pb_e_ = hipGetLastError();                                       \
+    if (pb_e_ != hipSuccess)                                                    \
+      return ::pb::set_error(PB_STATUS_DEVICE_ERROR, "kernel launch failed: %s " \
+                             "(%s:%d)", hipGetErrorString(pb_e_), __FILE__,     \
+                             __LINE__);                                         \
+  } while (0)
+
+#define PB_TRY(expr)                                                            \
+  do {                                                                          \
+    pb_status_t pb_s_ = (expr);                                                 \
+    if (pb_s_ != PB_STATUS_SUCCESS) return pb_s_;                               \
+  } while (0)
+
+#define PB_REQUIRE(cond, ...)                                                   \
+  do {                                                                          \
+    if (!(cond)) return ::pb::set_error(PB_STATUS_INVALID_VALUE, __VA_ARGS__);  \
+  } while (0)
+
+}  // namespace pb
+
+// ---------------------------------------------------------------------------
+// The handle: one stream, the device's properties, a growable workspace and a
+// small always-zero counter arena used by single-pass kernels (each kernel
+// returns the counters it used to zero before it exits, so no memset is ever
+// recorded into a stream or graph).
+// ---------------------------------------------------------------------------
+struct pb_selector_state {
+  int n_layers = 0;
+  std::vector<int> sizes;
+  std::vector<std::vector<float>> w, b;
+  std::vector<float> mean, stdv;
+};
+
+struct pb_handle_s {
+  hipStream_t stream = nullptr;
+  int device = 0;
+  int num_cus = 0;
+  int max_threads = 1024;
+  void* workspace = nullptr;
+  size_t workspace_bytes = 0;
+  unsigned int* counters = nullptr;  // kCounterSlots zero-initialised slots
+  void* scalar_slot = nullptr;       // 256 bytes for the _sync BLAS-1 results
+  size_t counter_cursor = 0;
+  bool capturing = false;
+  std::mutex mu;
+  pb_selector_state selector;
+};
+
+namespace pb {
+
+constexpr int kCounterSlots = 4096;
+
+// Validates the handle and makes its device current.
+pb_status_t enter(pb_handle_t h);
+// Returns a workspace of at least `bytes`, growing it if allowed.
+pb_status_t workspace(pb_handle_t h, size_t bytes, void** ptr);
+// Hands out `n` zeroed counter slots (round-robin over the arena).
+unsigned int* counters(pb_handle_t h, int n);
+
+inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+inline size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Lower-cased transpose flag; 'c' counts as 't' for real types
+// (src/interface/gemm_interface.hpp:141-151).  Returns 0 if invalid.
+inline char norm_trans(char t) {
+  char l = (t >= 'A' && t <= 'Z') ? char(t - 'A' + 'a') : t;
+  if (l == 'n') return 'n';
+  if (l == 't' || l == 'c') return 't';
+  return 0;
+}
+
+}  // namespace pb
+
+// ---------------------------------------------------------------------------
+// Device helpers
+// ---------------------------------------------------------------------------
+#if defined(__HIPCC__)
+namespace pbdev {
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+// Element i of a BLAS vector view (negative increments walk backwards from
+// the end, src/views/view_sycl.hpp:95-100).
+__device__ __forceinline__ int64_t vidx(int64_t i, int64_t n, int64_t inc) {
+  return inc >= 0 ? i * inc : (n - 1 - i) * (-inc);
+}
+
+}  // namespace pbdev
+#endif
