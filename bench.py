@@ -1,0 +1,378 @@
+"""Benchmark of the portBLAS hot path on one MI355X.
+
+Headline (the JSON `value`): FP32-accurate SGEMM at M = N = K = 8192 through
+BF16x3 split precision on MFMA (pb_gemm_ex, PB_GEMM_BF16X3), in FP32-equivalent
+TFLOP/s (2 M N K per step).  One step = one full call: the split/pack pre-pass
+and the 3-term BF16 MFMA GEMM, inputs resident in HBM.  The same JSON line
+carries the rest of the BASELINE metric and configs:
+
+* the plain FP32 MFMA SGEMM on the same inputs and the BF16x3 speedup;
+* the precision study: max / mean componentwise and normwise error of FP32
+  MFMA, FP32 SIMT, BF16x3, BF16 and FP8-e4m3 against the FP64 GPU GEMM
+  (pb_dgemm) on N(0,1) and log-uniform [2^-8, 2^8] inputs, and DGEMM 4096^3;
+* SGEMV 8192^2 'n'/'t' and sdot/sasum/snrm2/isamax at n = 2^26 in GB/s and as
+  a fraction of 8 TB/s (cache flushed before every timed call);
+* the strided batch (4096 x 64^3) and the grouped set (512 uneven BF16
+  problems, one persistent launch, direct and graph-replayed);
+* `roofline` for the dominant kernel (the BF16 MFMA GEMM of the BF16x3 step,
+  timed alone with HIP events on pre-split operands) and `cpu_baseline` (the
+  oracle's FP32 restatement of portBLAS's CPU GEMM schedule on the host).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0
+PEAK = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3, "fp64": 78.6}  # dense TFLOP/s
+METRIC = "SGEMM TFLOP/s, BF16x3-MFMA vs plain FP32 at equal FP64 error; GEMV/dot % HBM"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--quick", action="store_true", help="headline + roofline only")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    return ap.parse_args()
+
+
+class Timer:
+    """HIP-event timing on the current stream (the handle's stream)."""
+
+    def __init__(self, torch):
+        self.torch = torch
+
+    def __call__(self, fn, reps, warm=1, before=None):
+        torch = self.torch
+        for _ in range(warm):
+            if before:
+                before()
+            fn()
+        torch.cuda.synchronize()
+        total = 0.0
+        for _ in range(reps):
+            if before:
+                before()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            total += e0.elapsed_time(e1)
+        return total / reps  # ms
+
+
+def precision_study(torch, pb, h, timer, N, kind):
+    """Errors of every FP32-in variant against the FP64 GPU GEMM."""
+    g = torch.Generator(device="cuda").manual_seed(11 if kind == "normal" else 12)
+    if kind == "normal":
+        A = torch.randn(N * N, device="cuda", generator=g)
+        B = torch.randn(N * N, device="cuda", generator=g)
+    else:  # |x| log-uniform in [2^-8, 2^8], random signs
+        def lu():
+            mag = torch.exp2(torch.rand(N * N, device="cuda", generator=g) * 16 - 8)
+            sgn = torch.where(torch.rand(N * N, device="cuda", generator=g) < 0.5, -1.0, 1.0)
+            return (mag * sgn).float()
+        A, B = lu(), lu()
+    A64, B64 = A.double(), B.double()
+    C64 = torch.empty(N * N, dtype=torch.float64, device="cuda")
+    S64 = torch.empty(N * N, dtype=torch.float64, device="cuda")
+    pb._gemm(h, "n", "n", N, N, N, 1.0, A64, N, B64, N, 0.0, C64, N)
+    pb._gemm(h, "n", "n", N, N, N, 1.0, A64.abs(), N, B64.abs(), N, 0.0, S64, N)  # |A||B|
+    del A64, B64
+    nrm = torch.linalg.vector_norm(C64).item()
+    out = {}
+    C = torch.empty(N * N, device="cuda")
+    for name, prec in (("fp32_mfma", pb.gemm_precision_t.fp32_mfma),
+                       ("fp32_simt", pb.gemm_precision_t.fp32_simt),
+                       ("bf16x3", pb.gemm_precision_t.bf16x3),
+                       ("bf16", pb.gemm_precision_t.bf16),
+                       ("fp8_e4m3", pb.gemm_precision_t.fp8_e4m3)):
+        pb._gemm_ex(h, prec, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N)
+        d = (C.double() - C64).abs_()
+        comp = d / S64
+        out[name] = {
+            "max_rel": comp.max().item(),          # max |dC| / (|A||B|)
+            "mean_rel": comp.mean().item(),
+            "normwise": torch.linalg.vector_norm(d).item() / nrm,  # ||dC||_F / ||C||_F
+            "max_abs_over_max_c": d.max().item() / C64.abs().max().item(),
+        }
+        del d, comp
+    del C64, S64, C
+    torch.cuda.empty_cache()
+    return out
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build has no multi-GPU path "
+              f"(asked for --gpus {args.gpus}); run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+
+    import portblas_amd as pb
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    h = pb.SB_Handle()
+    timer = Timer(torch)
+    res: dict = {}
+
+    # ---------------- headline: BF16x3 SGEMM 8192^3 ----------------
+    N = 8192
+    flop = 2.0 * N ** 3
+    g = torch.Generator(device="cuda").manual_seed(0)
+    A = torch.rand(N * N, device="cuda", generator=g) * 2 - 1
+    B = torch.rand(N * N, device="cuda", generator=g) * 2 - 1
+    C = torch.zeros(N * N, device="cuda")
+    step = lambda: pb._gemm_ex(h, pb.gemm_precision_t.bf16x3, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = flop / (ms_per_step * 1e-3) / 1e12
+    log(f"bf16x3 sgemm {N}^3: {ms_per_step:.3f} ms/step, {value:.1f} TFLOP/s")
+
+    # plain FP32 MFMA on the same inputs
+    t_fp32 = timer(lambda: pb._gemm(h, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N), reps=3)
+    res["sgemm_fp32_mfma_tflops"] = flop / t_fp32 / 1e9
+    res["bf16x3_speedup_vs_fp32"] = value / res["sgemm_fp32_mfma_tflops"]
+    log(f"fp32 mfma: {t_fp32:.3f} ms, {res['sgemm_fp32_mfma_tflops']:.1f} TFLOP/s, "
+        f"speedup {res['bf16x3_speedup_vs_fp32']:.2f}x")
+
+    # roofline of the dominant kernel: the 3-term BF16 MFMA GEMM alone
+    L = pb.load()
+    kp = N
+    ahi = torch.empty(N * kp, dtype=torch.int16, device="cuda")
+    alo, bhi, blo = torch.empty_like(ahi), torch.empty_like(ahi), torch.empty_like(ahi)
+    pb.check(L.pb_pack_bf16x3(h.handle, 0, b"n", N, N, A.data_ptr(), N, ahi.data_ptr(), alo.data_ptr(), kp))
+    pb.check(L.pb_pack_bf16x3(h.handle, 1, b"n", N, N, B.data_ptr(), N, bhi.data_ptr(), blo.data_ptr(), kp))
+    kern = lambda: pb.check(L.pb_gemm_bf16x3_packed(h.handle, N, N, N, 1.0, ahi.data_ptr(), alo.data_ptr(), kp,
+                                                    bhi.data_ptr(), blo.data_ptr(), kp, 0.0, C.data_ptr(), N))
+    t_kern = timer(kern, reps=10, warm=2)
+    mfma_flop = 3 * flop
+    achieved = mfma_flop / (t_kern * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["bf16"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK["bf16"], "traffic": None,
+                "kernel": "gemm_kernel<bf16_t, 256x256x64 or selected tile> (3 BF16 terms)",
+                "kernel_ms": t_kern, "flop_per_launch": mfma_flop}
+    log(f"bf16 gemm kernel alone: {t_kern:.3f} ms, {achieved:.0f} TFLOP/s of BF16 MFMA "
+        f"({100 * achieved / PEAK['bf16']:.1f}% of peak)")
+    del ahi, alo, bhi, blo
+
+    if not args.quick:
+        # ---------------- precision study ----------------
+        res["precision_8192"] = {k: precision_study(torch, pb, h, timer, N, k) for k in ("normal", "loguniform")}
+        for kind, d in res["precision_8192"].items():
+            for name, e in d.items():
+                log(f"precision {kind:10s} {name:9s} max_rel {e['max_rel']:.3e} mean_rel {e['mean_rel']:.3e} "
+                    f"normwise {e['normwise']:.3e}")
+        # other variants' throughput on the headline inputs
+        for name, prec in (("bf16", pb.gemm_precision_t.bf16), ("fp8_e4m3", pb.gemm_precision_t.fp8_e4m3),
+                           ("fp32_simt", pb.gemm_precision_t.fp32_simt)):
+            t = timer(lambda: pb._gemm_ex(h, prec, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N), reps=3)
+            res[f"sgemm_{name}_tflops"] = flop / t / 1e9
+            log(f"{name}: {res[f'sgemm_{name}_tflops']:.1f} TFLOP/s")
+        del A, B, C
+        torch.cuda.empty_cache()
+        D = 4096
+        Ad = torch.rand(D * D, dtype=torch.float64, device="cuda")
+        Bd = torch.rand(D * D, dtype=torch.float64, device="cuda")
+        Cd = torch.zeros(D * D, dtype=torch.float64, device="cuda")
+        t = timer(lambda: pb._gemm(h, "n", "n", D, D, D, 1.0, Ad, D, Bd, D, 0.0, Cd, D), reps=3)
+        res["dgemm_4096_tflops"] = 2.0 * D ** 3 / t / 1e9
+        res["dgemm_4096_frac_fp64_peak"] = res["dgemm_4096_tflops"] / PEAK["fp64"]
+        log(f"dgemm 4096^3: {res['dgemm_4096_tflops']:.1f} TFLOP/s")
+        del Ad, Bd, Cd
+        M1 = 1024
+        A1 = torch.rand(M1 * M1, device="cuda") * 2 - 1
+        B1 = torch.rand(M1 * M1, device="cuda") * 2 - 1
+        C1 = torch.rand(M1 * M1, device="cuda") * 2 - 1
+        t = timer(lambda: pb._gemm(h, "n", "n", M1, M1, M1, 1.5, A1, M1, B1, M1, 0.5, C1, M1), reps=20)
+        res["sgemm_1024_tflops"] = 2.0 * M1 ** 3 / t / 1e9
+        log(f"sgemm 1024^3 (cfg1): {res['sgemm_1024_tflops']:.1f} TFLOP/s")
+
+        # ---------------- bandwidth-bound: GEMV and BLAS-1 ----------------
+        flushbuf = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        flush = lambda: flushbuf.fill_(1.0)
+        M = 8192
+        Ag = torch.rand(M * M, device="cuda") * 2 - 1
+        x = torch.rand(M, device="cuda") * 2 - 1
+        y = torch.zeros(M, device="cuda")
+        bw = {}
+        for tr in ("n", "t"):
+            t = timer(lambda: pb._gemv(h, tr, M, M, 1.0, Ag, M, x, 1, 0.0, y, 1), reps=10, before=flush)
+            gbs = 4.0 * (M * M + 2 * M) / t / 1e6
+            bw[f"sgemv_{tr}"] = {"GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "us": 1e3 * t}
+        del Ag
+        n = 1 << 26
+        v = torch.rand(n, device="cuda") * 2 - 1
+        w = torch.rand(n, device="cuda") * 2 - 1
+        rs = torch.zeros(4, device="cuda")
+        ri = torch.zeros(1, dtype=torch.int32, device="cuda")
+        for name, fn, nbytes in (("sdot", lambda: pb._dot(h, n, v, 1, w, 1, rs[0:1]), 8 * n),
+                                 ("sasum", lambda: pb._asum(h, n, v, 1, rs[1:2]), 4 * n),
+                                 ("snrm2", lambda: pb._nrm2(h, n, v, 1, rs[2:3]), 4 * n),
+                                 ("isamax", lambda: pb._iamax(h, n, v, 1, ri), 4 * n)):
+            t = timer(fn, reps=10, before=flush)
+            gbs = nbytes / t / 1e6
+            bw[name] = {"GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "us": 1e3 * t}
+        for k, e in bw.items():
+            log(f"{k:8s} {e['GB/s']:7.0f} GB/s  {100 * e['frac_hbm']:5.1f}% of HBM  ({e['us']:.1f} us)")
+        res["bandwidth"] = bw
+        del v, w
+
+        # ---------------- batched and grouped ----------------
+        bs, s = 4096, 64
+        Ab = torch.rand(bs * s * s, device="cuda") * 2 - 1
+        Bb = torch.rand(bs * s * s, device="cuda") * 2 - 1
+        Cb = torch.zeros(bs * s * s, device="cuda")
+        t = timer(lambda: pb._gemm_strided_batched(h, "n", "n", s, s, s, 1.0, Ab, s, s * s, Bb, s, s * s, 0.0,
+                                                   Cb, s, s * s, bs), reps=10, before=flush)
+        nbytes = 3.0 * bs * s * s * 4
+        res["strided_batched_4096x64"] = {"TFLOP/s": 2.0 * bs * s ** 3 / t / 1e9, "GB/s": nbytes / t / 1e6,
+                                          "frac_hbm": nbytes / t / 1e6 / HBM_PEAK_GBS, "us": 1e3 * t}
+        log(f"strided batched 4096x64^3: {res['strided_batched_4096x64']}")
+        del Ab, Bb, Cb, flushbuf
+        res["grouped_512_bf16"] = bench_grouped(torch, pb, h, timer)
+        log(f"grouped: {res['grouped_512_bf16']}")
+
+    # ---------------- CPU baseline ----------------
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(np, args.cpu_seconds)
+        log(f"cpu baseline: {cpu}")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "TFLOP/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "bf16x3 (fp32 in/out, bf16 MFMA, fp32 accumulate)",
+        "data": "synthetic (U[-1,1], seeded)",
+        "config": {"workload": "SGEMM 8192x8192x8192 col-major 'n','n' alpha 1 beta 0 via BF16x3 split precision",
+                   "M": N, "N": N, "K": N, "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "extra": res,
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_grouped(torch, pb, h, timer):
+    """512 problems, M, N, K log-uniform in [32, 1024] rounded to multiples of
+    8, BF16 inputs, FP32 accumulate/output, alpha 1, beta 0; one persistent
+    launch per call, also captured in a graph and replayed."""
+    import ctypes
+
+    import numpy as np
+
+    rng = np.random.default_rng(4)
+    dims = np.exp(rng.uniform(np.log(32), np.log(1024), size=(512, 3)))
+    dims = np.maximum(8, np.round(dims / 8) * 8).astype(np.int64)
+    L = pb.load()
+    bufs = []
+    probs = (ctypes.c_byte * 0)()
+
+    class Prob(ctypes.Structure):
+        _fields_ = [("transa", ctypes.c_char), ("transb", ctypes.c_char), ("m", ctypes.c_int64),
+                    ("n", ctypes.c_int64), ("k", ctypes.c_int64), ("alpha", ctypes.c_float),
+                    ("beta", ctypes.c_float), ("a", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                    ("b", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("c", ctypes.c_void_p),
+                    ("ldc", ctypes.c_int64)]
+
+    probs = (Prob * 512)()
+    flops = 0.0
+    for i, (m, n, k) in enumerate(dims):
+        a = (torch.rand(int(m * k), device="cuda") * 2 - 1).to(torch.bfloat16)
+        b = (torch.rand(int(k * n), device="cuda") * 2 - 1).to(torch.bfloat16)
+        c = torch.zeros(int(m * n), device="cuda")
+        bufs += [a, b, c]
+        probs[i] = Prob(b"n", b"n", int(m), int(n), int(k), 1.0, 0.0, a.data_ptr(), int(m), b.data_ptr(), int(k),
+                        c.data_ptr(), int(m))
+        flops += 2.0 * m * n * k
+    plan = ctypes.c_void_p()
+    pb.check(L.pb_gemm_grouped_plan_create(h.handle, int(pb.gemm_precision_t.bf16), probs, 512, ctypes.byref(plan)))
+    t = timer(lambda: pb.check(L.pb_gemm_grouped_plan_run(h.handle, plan)), reps=20, warm=3)
+    out = {"TFLOP/s": flops / t / 1e9, "us": 1e3 * t, "total_flop": flops}
+    # graph capture on a side stream, replayed
+    s = torch.cuda.Stream()
+    hs = pb.SB_Handle(s)
+    gph = ctypes.c_void_p()
+    with torch.cuda.stream(s):
+        pb.check(L.pb_graph_capture_begin(hs.handle))
+        pb.check(L.pb_gemm_grouped_plan_run(hs.handle, plan))
+        pb.check(L.pb_graph_capture_end(hs.handle, ctypes.byref(gph)))
+        tg = timer(lambda: pb.check(L.pb_graph_launch(hs.handle, gph)), reps=20, warm=3)
+    out["graph_TFLOP/s"] = flops / tg / 1e9
+    out["graph_us"] = 1e3 * tg
+    pb.check(L.pb_graph_destroy(gph))
+    pb.check(L.pb_gemm_grouped_plan_destroy(plan))
+    return out
+
+
+def cpu_baseline(np, seconds):
+    """portBLAS's CPU GEMM schedule restated in FP32 (oracle.pb_sgemm,
+    Tile<4,4,4,4> no-local, src/interface/blas3/backend/default.hpp:98-112) on
+    the reference's CPU-runnable config: 1024^3, alpha 1.5, beta 0.5, repeated
+    for about `seconds` seconds on the host's OpenMP threads."""
+    import oracle
+
+    M = 1024
+    A = oracle.uniform(0, M * M)
+    B = oracle.uniform(1, M * M)
+    C = oracle.uniform(2, M * M)
+    reps, t0 = 0, time.perf_counter()
+    while True:
+        oracle.pb_sgemm("n", "n", M, M, M, 1.5, A, M, B, M, 0.5, C, M)
+        reps += 1
+        el = time.perf_counter() - t0
+        if el >= seconds or reps >= 200:
+            break
+    tflops = 2.0 * M ** 3 * reps / el / 1e12
+    # CBLAS (OpenBLAS via numpy) on the same inputs, for context.
+    Am, Bm = A.reshape(M, M).T, B.reshape(M, M).T
+    t1 = time.perf_counter()
+    r2 = 0
+    while time.perf_counter() - t1 < min(seconds, 3.0):
+        _ = Am @ Bm
+        r2 += 1
+    cblas = 2.0 * M ** 3 * r2 / (time.perf_counter() - t1) / 1e12
+    return {"value": tflops, "unit": "TFLOP/s", "cores": oracle.num_threads(), "kind": "port",
+            "sample": f"SGEMM 1024^3 'n','n' alpha 1.5 beta 0.5 (portBLAS CPU schedule restated), {reps} calls "
+                      f"in {el:.1f} s", "cblas_openblas_tflops": cblas}
+
+
+if __name__ == "__main__":
+    main()

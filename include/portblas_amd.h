@@ -158,6 +158,20 @@ pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t precision, char transa
                        int64_t ldb, int64_t strideb, float beta, float* c,
                        int64_t ldc, int64_t stridec, int64_t batch_size);
 
+/* Split once, multiply many times (no reference counterpart): writes op(X)
+ * K-contiguous as BF16 hi = bf16(x) and lo = bf16(x - hi), row pitch ldp
+ * (>= k, multiple of 8).  operand 0: X is A and op(A) is rows x k;
+ * operand 1: X is B and op(B) is k x rows. */
+pb_status_t pb_pack_bf16x3(pb_handle_t h, int operand, char trans, int64_t rows, int64_t k,
+                           const float* x, int64_t ldx, uint16_t* hi, uint16_t* lo,
+                           int64_t ldp);
+/* C = alpha * (A_hi B_lo + A_lo B_hi + A_hi B_hi) + beta * C on packed
+ * operands from pb_pack_bf16x3 (the arithmetic of PB_GEMM_BF16X3). */
+pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k, float alpha,
+                                  const uint16_t* a_hi, const uint16_t* a_lo, int64_t ldpa,
+                                  const uint16_t* b_hi, const uint16_t* b_lo, int64_t ldpb,
+                                  float beta, float* c, int64_t ldc);
+
 /* BF16-in (raw uint16 bit patterns), FP32-accumulate, FP32-out GEMM. */
 pb_status_t pb_gemm_bf16(pb_handle_t h, char transa, char transb, int64_t m,
                          int64_t n, int64_t k, float alpha, const uint16_t* a,

@@ -11,13 +11,13 @@
 //          src/operations/blas1/IndexMaxMin.hpp:74-171,
 //          operators src/operations/blas_operators.hpp:276-316.
 //
-// MI355X design: one launch per call.  Each 256-thread workgroup streams its
-// share of the vector with 16-byte loads (4 in flight per thread), reduces in
+// MI355X design: each 256-thread workgroup streams its share of the vector
+// with 16-byte non-temporal loads (4 in flight per thread), reduces in
 // registers, across the wave with shuffles and across the 4 waves in LDS, and
-// writes one partial.  The last workgroup to finish (device-scope counter)
-// folds the partials in a fixed order, so the result is deterministic for a
-// given n (portBLAS's atomic order is not), and applies the same update to *rs
-// that portBLAS's atomics + epilogue produce.
+// writes one partial.  A one-workgroup fold kernel then combines the partials
+// in a fixed order, so the result is deterministic for a given n (portBLAS's
+// atomic order is not), and applies the same update to *rs that portBLAS's
+// atomics + epilogue produce.
 #include "pb_internal.hpp"
 
 #include <cfloat>
@@ -32,8 +32,8 @@ constexpr int kUnroll = 4;
 enum ReduceOp { kDot = 0, kAsum = 1, kNrm2 = 2 };
 
 template <typename T> struct Vec16;
-template <> struct Vec16<float> { using type = float4; static constexpr int n = 4; };
-template <> struct Vec16<double> { using type = double2; static constexpr int n = 2; };
+template <> struct Vec16<float> { using type = pbdev::f32x4_t; static constexpr int n = 4; };
+template <> struct Vec16<double> { using type = pbdev::f64x2_t; static constexpr int n = 2; };
 
 template <typename T, int OP>
 __device__ __forceinline__ T term(T a, T b) {
@@ -73,25 +73,11 @@ __device__ __forceinline__ T block_sum(T v) {
   return r;
 }
 
-// Returns true in every thread of the last workgroup to arrive.
-__device__ __forceinline__ bool arrive_last(unsigned int* counter) {
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    last = (prev == gridDim.x - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
 // VEC: both operands contiguous (inc == 1) and 16-byte aligned.
 template <typename T, int OP, bool VEC>
 __global__ __launch_bounds__(kBlock) void reduce_kernel(
     int64_t n, const T* __restrict__ x, int64_t incx, const T* __restrict__ y,
-    int64_t incy, T* __restrict__ partials, unsigned int* counter, T* rs) {
+    int64_t incy, T* __restrict__ partials) {
   T acc = T(0);
   if constexpr (VEC) {
     using V = typename Vec16<T>::type;
@@ -108,10 +94,10 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(
     for (; base + (kUnroll - 1) * kBlock < nvec; base += step) {
       V a[kUnroll], b[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) a[u] = xv[base + u * kBlock];
+      for (int u = 0; u < kUnroll; ++u) a[u] = pbdev::ld_stream(xv + base + u * kBlock);
       if constexpr (OP == kDot) {
 #pragma unroll
-        for (int u = 0; u < kUnroll; ++u) b[u] = yv[base + u * kBlock];
+        for (int u = 0; u < kUnroll; ++u) b[u] = pbdev::ld_stream(yv + base + u * kBlock);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
@@ -123,8 +109,8 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(
       for (int u = 0; u < kUnroll; ++u) {
         const int64_t i = base + u * kBlock;
         if (i < nvec) {
-          V a = xv[i];
-          V b = (OP == kDot) ? yv[i] : a;
+          V a = pbdev::ld_stream(xv + i);
+          V b = (OP == kDot) ? pbdev::ld_stream(yv + i) : a;
           acc_u[u] = vterm_sum<T, OP>(a, b, acc_u[u]);
         }
       }
@@ -139,22 +125,28 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(
   } else {
     const int64_t step = int64_t(gridDim.x) * kBlock;
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += step) {
-      const T a = x[pbdev::vidx(i, n, incx)];
-      const T b = (OP == kDot) ? y[pbdev::vidx(i, n, incy)] : a;
+      const T a = pbdev::ld_stream(x + pbdev::vidx(i, n, incx));
+      const T b = (OP == kDot) ? pbdev::ld_stream(y + pbdev::vidx(i, n, incy)) : a;
       acc += term<T, OP>(a, b);
     }
   }
   T bsum = block_sum(acc);
   if (threadIdx.x == 0) partials[blockIdx.x] = bsum;
-  if (!arrive_last(counter)) return;
-  // Fixed-order fold of the partials by the last workgroup.
+}
+
+// Fixed-order fold of the per-workgroup partials (a second, one-workgroup
+// launch: stream order makes the partials visible without device-scope
+// fences, which on MI355X would write back the L2 once per workgroup) and the
+// update of *rs that portBLAS's atomics + sqrt epilogue produce.
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void reduce_fold_kernel(const T* __restrict__ partials,
+                                                             int nparts, T* rs) {
   T s = T(0);
-  for (int i = threadIdx.x; i < int(gridDim.x); i += kBlock) s += partials[i];
-  T total = block_sum(s);
+  for (int i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
+  const T total = block_sum(s);
   if (threadIdx.x == 0) {
     if constexpr (OP == kNrm2) rs[0] = sqrt(rs[0] + total);
     else rs[0] = rs[0] + total;
-    *counter = 0u;  // leave the arena zeroed for the next kernel
   }
 }
 
@@ -204,7 +196,7 @@ __device__ __forceinline__ void block_best(T& v, int64_t& i) {
 template <bool IS_MAX, typename T, typename IndexT, bool VEC>
 __global__ __launch_bounds__(kBlock) void iamax_kernel(
     int64_t n, const T* __restrict__ x, int64_t incx, T* __restrict__ pv,
-    int64_t* __restrict__ pi, unsigned int* counter, IndexT* rs) {
+    int64_t* __restrict__ pi) {
   T bv = init_val<IS_MAX, T>();
   int64_t bi = kSentinel;
   if constexpr (VEC) {
@@ -219,16 +211,15 @@ __global__ __launch_bounds__(kBlock) void iamax_kernel(
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const int64_t iv = base + u * kBlock;
-        if (iv < nvec) a[u] = xv[iv];
+        if (iv < nvec) a[u] = pbdev::ld_stream(xv + iv);
       }
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const int64_t iv = base + u * kBlock;
         if (iv < nvec) {
-          const T* e = reinterpret_cast<const T*>(&a[u]);
 #pragma unroll
           for (int j = 0; j < VN; ++j) {
-            const T v = fabs(e[j]);
+            const T v = fabs(a[u][j]);
             const int64_t idx = iv * VN + j;
             if (better<IS_MAX>(v, idx, bv, bi)) { bv = v; bi = idx; }
           }
@@ -243,22 +234,25 @@ __global__ __launch_bounds__(kBlock) void iamax_kernel(
   } else {
     const int64_t step = int64_t(gridDim.x) * kBlock;
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += step) {
-      const T v = fabs(x[i * incx]);
+      const T v = fabs(pbdev::ld_stream(x + i * incx));
       if (better<IS_MAX>(v, i, bv, bi)) { bv = v; bi = i; }
     }
   }
   block_best<IS_MAX>(bv, bi);
   if (threadIdx.x == 0) { pv[blockIdx.x] = bv; pi[blockIdx.x] = bi; }
-  if (!arrive_last(counter)) return;
-  bv = init_val<IS_MAX, T>();
-  bi = kSentinel;
-  for (int w = threadIdx.x; w < int(gridDim.x); w += kBlock)
+}
+
+template <bool IS_MAX, typename T, typename IndexT>
+__global__ __launch_bounds__(kBlock) void iamax_fold_kernel(const T* __restrict__ pv,
+                                                            const int64_t* __restrict__ pi,
+                                                            int nparts, IndexT* rs) {
+  T bv = init_val<IS_MAX, T>();
+  int64_t bi = kSentinel;
+  for (int w = threadIdx.x; w < nparts; w += kBlock)
     if (better<IS_MAX>(pv[w], pi[w], bv, bi)) { bv = pv[w]; bi = pi[w]; }
   block_best<IS_MAX>(bv, bi);
-  if (threadIdx.x == 0) {
+  if (threadIdx.x == 0)
     rs[0] = (bi == kSentinel) ? std::numeric_limits<IndexT>::max() : IndexT(bi);
-    *counter = 0u;
-  }
 }
 
 template <typename IndexT>
@@ -269,7 +263,7 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 int grid_for(pb_handle_t h, int64_t n, int elems_per_thread_iter) {
   const int64_t per_block = int64_t(kBlock) * elems_per_thread_iter;
   int64_t g = pb::ceil_div(n, per_block);
-  const int64_t cap = int64_t(h->num_cus) * 8;
+  const int64_t cap = int64_t(h->num_cus) * 4;
   if (g > cap) g = cap;
   if (g < 1) g = 1;
   return int(g);
@@ -297,13 +291,15 @@ pb_status_t reduce_launch(pb_handle_t h, int64_t n, const T* x, int64_t incx,
   const int grid = vec ? grid_for(h, n, VN * kUnroll) : grid_for(h, n, 4);
   void* ws = nullptr;
   PB_TRY(pb::workspace(h, sizeof(T) * grid, &ws));
-  unsigned int* ctr = pb::counters(h, 1);
+  T* partials = static_cast<T*>(ws);
   if (vec)
     hipLaunchKernelGGL((reduce_kernel<T, OP, true>), dim3(grid), dim3(kBlock), 0,
-                       h->stream, n, x, incx, y, incy, static_cast<T*>(ws), ctr, rs);
+                       h->stream, n, x, incx, y, incy, partials);
   else
     hipLaunchKernelGGL((reduce_kernel<T, OP, false>), dim3(grid), dim3(kBlock), 0,
-                       h->stream, n, x, incx, y, incy, static_cast<T*>(ws), ctr, rs);
+                       h->stream, n, x, incx, y, incy, partials);
+  hipLaunchKernelGGL((reduce_fold_kernel<T, OP>), dim3(1), dim3(kBlock), 0, h->stream, partials,
+                     grid, rs);
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }
@@ -338,13 +334,14 @@ pb_status_t iamax_launch(pb_handle_t h, int64_t n, const T* x, int64_t incx,
   PB_TRY(pb::workspace(h, (sizeof(T) + sizeof(int64_t)) * grid + 256, &ws));
   int64_t* pi = static_cast<int64_t*>(ws);
   T* pv = reinterpret_cast<T*>(pi + grid);
-  unsigned int* ctr = pb::counters(h, 1);
   if (vec)
     hipLaunchKernelGGL((iamax_kernel<IS_MAX, T, IndexT, true>), dim3(grid),
-                       dim3(kBlock), 0, h->stream, n, x, incx, pv, pi, ctr, rs);
+                       dim3(kBlock), 0, h->stream, n, x, incx, pv, pi);
   else
     hipLaunchKernelGGL((iamax_kernel<IS_MAX, T, IndexT, false>), dim3(grid),
-                       dim3(kBlock), 0, h->stream, n, x, incx, pv, pi, ctr, rs);
+                       dim3(kBlock), 0, h->stream, n, x, incx, pv, pi);
+  hipLaunchKernelGGL((iamax_fold_kernel<IS_MAX, T, IndexT>), dim3(1), dim3(kBlock), 0, h->stream,
+                     pv, pi, grid, rs);
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }

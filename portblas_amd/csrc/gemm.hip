@@ -174,9 +174,13 @@ inline Plan plan_plain(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t b
   const int64_t cus = h->num_cus;
   const int64_t t128 = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
   const int64_t t64 = pb::ceil_div(M, 64) * pb::ceil_div(N, 64) * batch;
-  if (t128 >= 2 * cus) return {true, 1};
-  if (t64 >= 2 * cus) return {false, 1};
-  int64_t s = pb::ceil_div(2 * cus, t64);
+  // Padded area of the 128 x 128 tiling relative to the problem: large tiles
+  // only when they are nearly full (otherwise every tile runs guarded).
+  const double pad128 = double(pb::ceil_div(M, 128) * 128) * double(pb::ceil_div(N, 128) * 128) /
+                        (double(M) * double(N));
+  if (t128 >= 2 * cus && pad128 < 1.15) return {true, 1};
+  if (t64 >= cus) return {false, 1};
+  int64_t s = pb::ceil_div(cus, t64);
   const int64_t smax = K / (4 * bk);
   if (s > smax) s = smax;
   if (s < 1) s = 1;

@@ -92,10 +92,34 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmProblem<T> p) {
     constexpr int64_t TE = int64_t(TL::BM) * TL::BN;
     Acc* part = p.partial + (b * tiles + tid) * p.splits * TE;
     TL::store_partial(acc, part + s * TE);
-    if (!arrive_last(p.counters + (b * tiles + tid), unsigned(p.splits))) continue;
-    TL::zero(acc);
-    for (int s2 = 0; s2 < p.splits; ++s2) TL::add_partial(acc, part + s2 * TE);
-    TL::epilogue(acc, alpha, p.beta, c, p.c_rs, p.ldc, p.M, p.N, m0, n0, p.c_vec);
+  }
+}
+
+// Split-K: sums the K-split partial tiles in a fixed order and applies the
+// epilogue.  A separate launch (stream order publishes the partials; no
+// device-scope fences, which would write back the L2 once per workgroup).
+// Grid: (tiles, batch); element e of a tile is (e % BM, e / BM), so
+// consecutive threads walk the contiguous rows of C.
+template <typename T, int BM, int BN>
+__global__ __launch_bounds__(256) void splitk_fold_kernel(GemmProblem<T> p) {
+  using Acc = typename Traits<T>::Acc;
+  constexpr int64_t TE = int64_t(BM) * BN;
+  const int64_t tiles = p.tiles_m * p.tiles_n;
+  const int64_t tid = blockIdx.x;
+  const int64_t tm = tid / p.tiles_n, tn = tid % p.tiles_n;
+  for (int64_t b = blockIdx.y; b < p.batch; b += gridDim.y) {
+    const Acc* part = p.partial + (b * tiles + tid) * p.splits * TE;
+    Acc* c = p.c + b * p.c_stride;
+    const Acc alpha = p.alpha_dev ? p.alpha * *p.alpha_dev : p.alpha;
+    for (int e = threadIdx.x; e < TE; e += 256) {
+      const int64_t m = tm * BM + e % BM, n = tn * BN + e / BM;
+      if (m >= p.M || n >= p.N) continue;
+      Acc sum = part[e];
+      for (int s = 1; s < p.splits; ++s) sum += part[s * TE + e];
+      Acc* cp = c + m * p.c_rs + n * p.ldc;
+      const Acc v = alpha * sum;
+      *cp = (p.beta != Acc(0)) ? v + p.beta * *cp : v;
+    }
   }
 }
 
@@ -129,7 +153,6 @@ pb_status_t launch_gemm(pb_handle_t h, GemmProblem<T> p, bool kca, bool kcb, boo
   if (splits < 1) splits = 1;
   const int64_t max_splits = pb::ceil_div(p.K, BK);
   if (splits > max_splits) splits = int(max_splits < 1 ? 1 : max_splits);
-  if (splits > 1 && tiles * p.batch > pb::kCounterSlots) splits = 1;
   p.kchunk = pb::ceil_div(pb::ceil_div(p.K, splits), BK) * BK;
   if (p.kchunk < BK) p.kchunk = BK;
   splits = int(pb::ceil_div(p.K, p.kchunk));
@@ -154,6 +177,11 @@ pb_status_t launch_gemm(pb_handle_t h, GemmProblem<T> p, bool kca, bool kcb, boo
   if (edge > 0) {
     p.tile_set = vec ? kEdgeTiles : kAllTiles;
     dispatch_layout<T, Shape, true>(h, p, dim3{unsigned(edge * splits), gy}, kca, kcb);
+    PB_CHECK_LAUNCH();
+  }
+  if (splits > 1) {
+    const dim3 fold_grid{unsigned(tiles), gy};
+    hipLaunchKernelGGL((splitk_fold_kernel<T, BM, BN>), fold_grid, dim3(256), 0, h->stream, p);
     PB_CHECK_LAUNCH();
   }
   return PB_STATUS_SUCCESS;

@@ -362,6 +362,58 @@ pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t prec, char ta, char tb
   return gemm_ex_lowp(h, prec, g);
 }
 
+pb_status_t pb_pack_bf16x3(pb_handle_t h, int operand, char trans, int64_t rows, int64_t k,
+                           const float* x, int64_t ldx, uint16_t* hi, uint16_t* lo,
+                           int64_t ldp) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  const char t = pb::norm_trans(trans);
+  PB_REQUIRE(t != 0, "invalid transpose flag");
+  PB_REQUIRE(operand == 0 || operand == 1, "operand must be 0 (A) or 1 (B)");
+  PB_REQUIRE(rows >= 0 && k >= 0, "negative dimension");
+  if (rows == 0 || k == 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(x && hi && lo, "null pointer");
+  PB_REQUIRE(ldp >= k && ldp % 8 == 0, "ldp must be >= k and a multiple of 8");
+  // A: op(A) is rows x k; B: op(B) is k x rows, addressed as (row, k).
+  int64_t rs, ks;
+  if (operand == 0) { rs = t == 'n' ? 1 : ldx; ks = t == 'n' ? ldx : 1; }
+  else { rs = t == 'n' ? ldx : 1; ks = t == 'n' ? 1 : ldx; }
+  PB_REQUIRE(ldx >= ((rs == 1) ? rows : k), "leading dimension too small");
+  dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(ldp, 64)), 1u};
+  hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks, rows,
+                     k, int64_t(0), ldp, static_cast<void*>(hi), static_cast<void*>(lo),
+                     static_cast<const float*>(nullptr), 0);
+  PB_CHECK_LAUNCH();
+  return PB_STATUS_SUCCESS;
+}
+
+pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k, float alpha,
+                                  const uint16_t* a_hi, const uint16_t* a_lo, int64_t ldpa,
+                                  const uint16_t* b_hi, const uint16_t* b_lo, int64_t ldpb,
+                                  float beta, float* c, int64_t ldc) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  PB_REQUIRE(m >= 0 && n >= 0 && k >= 0, "negative dimension");
+  if (m == 0 || n == 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(c != nullptr && ldc >= m, "bad C");
+  PB_REQUIRE(k == 0 || (a_hi && a_lo && b_hi && b_lo), "null operand");
+  PB_REQUIRE(ldpa >= k && ldpb >= k, "packed pitch smaller than k");
+  if (alpha == 0.f) return sgemm_alpha_zero(h, m, n, beta, c, ldc, 0, 1);
+  GemmProblem<bf16_t> p{};
+  p.M = m; p.N = n; p.K = k; p.alpha = alpha; p.beta = beta;
+  p.nterms = 3;
+  p.a[0] = a_hi; p.b[0] = b_lo;
+  p.a[1] = a_lo; p.b[1] = b_hi;
+  p.a[2] = a_hi; p.b[2] = b_hi;
+  p.a_rs = ldpa; p.a_ks = 1; p.a_stride = 0;
+  p.b_rs = ldpb; p.b_ks = 1; p.b_stride = 0;
+  p.c = c; p.c_rs = 1; p.ldc = ldc; p.c_stride = 0; p.batch = 1;
+  p.c_vec = aligned16(c) && ldc % 4 == 0;
+  const bool vec = ldpa % 8 == 0 && ldpb % 8 == 0 && aligned16(a_hi) && aligned16(a_lo) &&
+                   aligned16(b_hi) && aligned16(b_lo);
+  return run_lowp<bf16_t>(h, -1, p, true, true, vec);
+}
+
 pb_status_t pb_gemm_bf16(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
                          float alpha, const uint16_t* a, int64_t lda, int64_t sa,
                          const uint16_t* b, int64_t ldb, int64_t sb, float beta, float* c,

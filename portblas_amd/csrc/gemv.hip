@@ -6,13 +6,13 @@
 // ld x WGs_per_C matrix, then SumMatrixColumns + the epilogue
 // y = beta*y + alpha*sum (beta == 0 -> y not read, :185-212).
 //
-// MI355X design (one launch, HBM-bound):
+// MI355X design (HBM-bound; A is read once with non-temporal 16-byte loads):
 //  'n'  a workgroup of 4 waves owns 256 consecutive rows; every lane holds 4
 //       consecutive rows (one 16-byte load per column for float) and the 4
 //       waves split the workgroup's column chunk.  Partial sums are folded
-//       across waves in LDS; column chunks write partials to the workspace and
-//       the last chunk to finish for a row tile (device-scope counter) folds
-//       them in a fixed order and applies the epilogue.
+//       across waves in LDS; with several column chunks, each writes its
+//       partials to the workspace and a fold kernel sums them in a fixed
+//       order and applies the epilogue.
 //  't'  a workgroup owns a 2048-row chunk and a group of columns; each lane
 //       keeps its 32 elements of x in registers for the whole workgroup, each
 //       wave walks its columns with 16-byte loads and reduces across the wave
@@ -23,38 +23,25 @@ namespace {
 
 constexpr int kBlock = 256;
 constexpr int kWaves = kBlock / 64;
-constexpr int kRowsPerTile = 256;   // 'n': rows per workgroup (64 lanes x 4)
 constexpr int kTQ = 8;              // 't': 16-byte row groups per lane
 constexpr int kTRows = 64 * 4 * kTQ;  // 't': rows per chunk (2048)
 
 template <typename T> struct Four { T v[4]; };
 
-template <typename T>
-__device__ __forceinline__ Four<T> load4(const T* p);
-template <>
-__device__ __forceinline__ Four<float> load4<float>(const float* p) {
-  float4 a = *reinterpret_cast<const float4*>(p);
-  return {{a.x, a.y, a.z, a.w}};
-}
-template <>
-__device__ __forceinline__ Four<double> load4<double>(const double* p) {
-  double2 a = reinterpret_cast<const double2*>(p)[0];
-  double2 b = reinterpret_cast<const double2*>(p)[1];
-  return {{a.x, a.y, b.x, b.y}};
-}
-
-__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int total) {
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    last = (prev == total - 1);
-    if (last) *counter = 0u;  // every other arrival for this tile is done
+// Four consecutive elements; STREAM selects a non-temporal load (A is read
+// once; x is shared by every workgroup and stays cacheable).
+template <typename T, bool STREAM = true>
+__device__ __forceinline__ Four<T> load4(const T* p) {
+  if constexpr (sizeof(T) == 4) {
+    const auto* q = reinterpret_cast<const pbdev::f32x4_t*>(p);
+    const pbdev::f32x4_t a = STREAM ? pbdev::ld_stream(q) : *q;
+    return {{a.x, a.y, a.z, a.w}};
+  } else {
+    const auto* q = reinterpret_cast<const pbdev::f64x2_t*>(p);
+    const pbdev::f64x2_t a = STREAM ? pbdev::ld_stream(q) : q[0];
+    const pbdev::f64x2_t b = STREAM ? pbdev::ld_stream(q + 1) : q[1];
+    return {{a.x, a.y, b.x, b.y}};
   }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
 }
 
 template <typename T>
@@ -63,84 +50,105 @@ __device__ __forceinline__ void store_y(T* y, int64_t idx, T alpha, T beta, T su
 }
 
 // ---------------------------------------------------------------------------
-// 'n': grid (row_tiles, col_chunks).  VEC: A 16-byte aligned with lda a
-// multiple of 4 elements, so lane rows [4l, 4l+4) load as one vector.
+// 'n': grid (row_chunks, col_chunks).  A workgroup owns kTRows (2048) rows;
+// every wave walks its columns (the 4 waves interleave them) and reads each
+// column's 2048-row segment as one contiguous 8 KB stream, lane rows
+// r0 + 256 q + 4 lane + e (VEC: A 16-byte aligned with lda a multiple of 4
+// elements) or r0 + 64 (4 q + e) + lane (scalar).  The 4 waves' sums are
+// folded in LDS in a fixed order.
 // ---------------------------------------------------------------------------
 template <typename T, bool VEC>
 __global__ __launch_bounds__(kBlock) void gemv_n_kernel(
     int64_t M, int64_t N, T alpha, const T* __restrict__ A, int64_t lda,
     const T* __restrict__ x, int64_t incx, T beta, T* y, int64_t incy,
-    int64_t cols_per_chunk, T* __restrict__ partials, unsigned int* counters) {
-  __shared__ T red[kWaves][kRowsPerTile + 4];
+    int64_t cols_per_chunk, T* __restrict__ partials) {
+  __shared__ T red[kWaves][kTRows];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t row0 = int64_t(blockIdx.x) * kRowsPerTile;
+  const int64_t r0 = int64_t(blockIdx.x) * kTRows;
   const int64_t c0 = int64_t(blockIdx.y) * cols_per_chunk;
   int64_t c1 = c0 + cols_per_chunk;
   if (c1 > N) c1 = N;
-  T acc[4] = {T(0), T(0), T(0), T(0)};
-  // Row r of this lane: VEC -> row0 + 4*lane + q; scalar -> row0 + lane + 64*q.
-  const bool full = row0 + kRowsPerTile <= M;
-  if (VEC && full) {
-    const T* Ap = A + row0 + 4 * lane;
-    int64_t j = c0 + wave;
-    // 4 columns per step, each wave strides by kWaves columns.
-    for (; j + 3 * kWaves < c1; j += 4 * kWaves) {
-      Four<T> a0 = load4(Ap + (j + 0 * kWaves) * lda);
-      Four<T> a1 = load4(Ap + (j + 1 * kWaves) * lda);
-      Four<T> a2 = load4(Ap + (j + 2 * kWaves) * lda);
-      Four<T> a3 = load4(Ap + (j + 3 * kWaves) * lda);
-      const T x0 = x[pbdev::vidx(j + 0 * kWaves, N, incx)];
-      const T x1 = x[pbdev::vidx(j + 1 * kWaves, N, incx)];
-      const T x2 = x[pbdev::vidx(j + 2 * kWaves, N, incx)];
-      const T x3 = x[pbdev::vidx(j + 3 * kWaves, N, incx)];
+  T acc[kTQ][4];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        acc[q] += a0.v[q] * x0;
-        acc[q] += a1.v[q] * x1;
-        acc[q] += a2.v[q] * x2;
-        acc[q] += a3.v[q] * x3;
-      }
+  for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[q][e] = T(0);
+  const bool full = r0 + kTRows <= M;
+  if (VEC && full) {
+    const T* Ap = A + r0 + 4 * lane;
+    int64_t j = c0 + wave;
+    // Two columns per step: 2 x kTQ 16-byte loads in flight per lane.
+    for (; j + kWaves < c1; j += 2 * kWaves) {
+      Four<T> a[kTQ], b[kTQ];
+#pragma unroll
+      for (int q = 0; q < kTQ; ++q) a[q] = load4(Ap + j * lda + 256 * q);
+#pragma unroll
+      for (int q = 0; q < kTQ; ++q) b[q] = load4(Ap + (j + kWaves) * lda + 256 * q);
+      const T xa = x[pbdev::vidx(j, N, incx)];
+      const T xb = x[pbdev::vidx(j + kWaves, N, incx)];
+#pragma unroll
+      for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          acc[q][e] += a[q].v[e] * xa;
+          acc[q][e] += b[q].v[e] * xb;
+        }
     }
     for (; j < c1; j += kWaves) {
-      Four<T> a0 = load4(Ap + j * lda);
-      const T x0 = x[pbdev::vidx(j, N, incx)];
+      const T xa = x[pbdev::vidx(j, N, incx)];
 #pragma unroll
-      for (int q = 0; q < 4; ++q) acc[q] += a0.v[q] * x0;
+      for (int q = 0; q < kTQ; ++q) {
+        const Four<T> a = load4(Ap + j * lda + 256 * q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[q][e] += a.v[e] * xa;
+      }
     }
   } else {
     for (int64_t j = c0 + wave; j < c1; j += kWaves) {
       const T xj = x[pbdev::vidx(j, N, incx)];
+      const T* col = A + j * lda;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t r = VEC ? row0 + 4 * lane + q : row0 + lane + 64 * q;
-        if (r < M) acc[q] += A[r + j * lda] * xj;
-      }
+      for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t r = VEC ? r0 + 256 * q + 4 * lane + e : r0 + 64 * (4 * q + e) + lane;
+          if (r < M) acc[q][e] += pbdev::ld_stream(col + r) * xj;
+        }
     }
   }
-  // Fold the 4 waves in LDS (fixed order).
+  // Fold the 4 waves in LDS (fixed order), one row per thread per pass.
 #pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const int rr = VEC ? 4 * lane + q : lane + 64 * q;
-    red[wave][rr] = acc[q];
-  }
+  for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      const int rr = VEC ? 256 * q + 4 * lane + e : 64 * (4 * q + e) + lane;
+      red[wave][rr] = acc[q][e];
+    }
   __syncthreads();
-  const int rr = threadIdx.x;  // one row of the tile per thread
-  T s = red[0][rr];
+  for (int rr = threadIdx.x; rr < kTRows; rr += kBlock) {
+    T s = red[0][rr];
 #pragma unroll
-  for (int w = 1; w < kWaves; ++w) s += red[w][rr];
-  const int64_t r = row0 + rr;
-  if (gridDim.y == 1) {
-    if (r < M) store_y(y, pbdev::vidx(r, M, incy), alpha, beta, s);
-    return;
+    for (int w = 1; w < kWaves; ++w) s += red[w][rr];
+    const int64_t r = r0 + rr;
+    if (r >= M) continue;
+    if (gridDim.y == 1) store_y(y, pbdev::vidx(r, M, incy), alpha, beta, s);
+    else partials[int64_t(blockIdx.y) * M + r] = s;
   }
-  if (r < M) partials[int64_t(blockIdx.y) * M + r] = s;
-  if (!arrive_last(counters + blockIdx.x, gridDim.y)) return;
-  if (r < M) {
-    T t = T(0);
-    for (unsigned c = 0; c < gridDim.y; ++c) t += partials[int64_t(c) * M + r];
-    store_y(y, pbdev::vidx(r, M, incy), alpha, beta, t);
-  }
+}
+
+// Folds the per-chunk partial sums in a fixed order and applies the epilogue
+// (a separate launch: stream order publishes the partials without
+// device-scope fences).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void gemv_fold_kernel(int64_t len, int chunks,
+                                                           const T* __restrict__ partials,
+                                                           T alpha, T beta, T* y, int64_t incy) {
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= len) return;
+  T t = T(0);
+  for (int c = 0; c < chunks; ++c) t += partials[int64_t(c) * len + i];
+  store_y(y, pbdev::vidx(i, len, incy), alpha, beta, t);
 }
 
 // ---------------------------------------------------------------------------
@@ -151,7 +159,7 @@ template <typename T, bool VEC>
 __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
     int64_t M, int64_t N, T alpha, const T* __restrict__ A, int64_t lda,
     const T* __restrict__ x, int64_t incx, T beta, T* y, int64_t incy,
-    int64_t cols_per_group, T* __restrict__ partials, unsigned int* counters) {
+    int64_t cols_per_group, T* __restrict__ partials) {
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
   const int64_t r0 = int64_t(blockIdx.y) * kTRows;
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
 #pragma unroll
   for (int q = 0; q < kTQ; ++q) {
     if (VEC && full) {
-      Four<T> v = load4(x + r0 + 256 * q + 4 * lane);
+      Four<T> v = load4<T, false>(x + r0 + 256 * q + 4 * lane);
 #pragma unroll
       for (int e = 0; e < 4; ++e) xr[q][e] = v.v[e];
     } else {
@@ -193,7 +201,7 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
 #pragma unroll
         for (int e = 0; e < 4; ++e) {
           const int64_t r = VEC ? r0 + 256 * q + 4 * lane + e : r0 + 64 * (4 * q + e) + lane;
-          if (r < M) acc += col[r] * xr[q][e];
+          if (r < M) acc += pbdev::ld_stream(col + r) * xr[q][e];
         }
     }
     acc = pbdev::wave_sum(acc);
@@ -201,13 +209,6 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
       if (gridDim.y == 1) store_y(y, pbdev::vidx(j, N, incy), alpha, beta, acc);
       else partials[int64_t(blockIdx.y) * N + j] = acc;
     }
-  }
-  if (gridDim.y == 1) return;
-  if (!arrive_last(counters + blockIdx.x, gridDim.y)) return;
-  for (int64_t j = j0 + threadIdx.x; j < j1; j += kBlock) {
-    T t = T(0);
-    for (unsigned c = 0; c < gridDim.y; ++c) t += partials[int64_t(c) * N + j];
-    store_y(y, pbdev::vidx(j, N, incy), alpha, beta, t);
   }
 }
 
@@ -250,27 +251,29 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
   PB_REQUIRE(lda >= (M > 1 ? M : 1), "lda (%lld) < max(1, M)", (long long)lda);
   const bool avec = aligned16(A) && (lda * int64_t(sizeof(T))) % 16 == 0;
   const int target = 4 * h->num_cus;
-  unsigned int* ctr = pb::counters(h, 0);
   if (!tr) {
-    const int64_t tiles = pb::ceil_div(M, kRowsPerTile);
-    PB_REQUIRE(tiles <= 65535 * 4096LL, "M too large");
+    const int64_t tiles = pb::ceil_div(M, kTRows);
+    PB_REQUIRE(tiles <= 0x7fffffffLL, "M too large");
     int64_t chunks = pb::ceil_div(target, tiles);
     const int64_t max_chunks = pb::ceil_div(N, 4 * kWaves);  // >= 16 columns per chunk
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
-    if (tiles > pb::kCounterSlots) chunks = 1;  // one pass, no cross-chunk fold
     if (chunks > 65535) chunks = 65535;
     const int64_t cpc = pb::ceil_div(N, chunks);
     chunks = pb::ceil_div(N, cpc);
     void* ws = nullptr;
     if (chunks > 1) PB_TRY(pb::workspace(h, sizeof(T) * size_t(chunks) * size_t(M), &ws));
+    T* part = static_cast<T*>(ws);
     dim3 grid{unsigned(tiles), unsigned(chunks)};
     if (avec)
       hipLaunchKernelGGL((gemv_n_kernel<T, true>), grid, dim3(kBlock), 0, h->stream, M, N,
-                         alpha, A, lda, x, incx, beta, y, incy, cpc, static_cast<T*>(ws), ctr);
+                         alpha, A, lda, x, incx, beta, y, incy, cpc, part);
     else
       hipLaunchKernelGGL((gemv_n_kernel<T, false>), grid, dim3(kBlock), 0, h->stream, M, N,
-                         alpha, A, lda, x, incx, beta, y, incy, cpc, static_cast<T*>(ws), ctr);
+                         alpha, A, lda, x, incx, beta, y, incy, cpc, part);
+    if (chunks > 1)
+      hipLaunchKernelGGL(gemv_fold_kernel<T>, dim3(unsigned(pb::ceil_div(M, kBlock))),
+                         dim3(kBlock), 0, h->stream, M, int(chunks), part, alpha, beta, y, incy);
   } else {
     const bool vec = avec && incx == 1 && aligned16(x);
     const int64_t chunks = pb::ceil_div(M, kTRows);
@@ -280,18 +283,21 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
     const int64_t max_groups = pb::ceil_div(N, kWaves);
     if (groups > max_groups) groups = max_groups;
     if (groups < 1) groups = 1;
-    if (chunks > 1 && groups > pb::kCounterSlots) groups = pb::kCounterSlots;
     const int64_t cpg = pb::ceil_div(N, groups);
     groups = pb::ceil_div(N, cpg);
     void* ws = nullptr;
     if (chunks > 1) PB_TRY(pb::workspace(h, sizeof(T) * size_t(chunks) * size_t(N), &ws));
+    T* part = static_cast<T*>(ws);
     dim3 grid{unsigned(groups), unsigned(chunks)};
     if (vec)
       hipLaunchKernelGGL((gemv_t_kernel<T, true>), grid, dim3(kBlock), 0, h->stream, M, N,
-                         alpha, A, lda, x, incx, beta, y, incy, cpg, static_cast<T*>(ws), ctr);
+                         alpha, A, lda, x, incx, beta, y, incy, cpg, part);
     else
       hipLaunchKernelGGL((gemv_t_kernel<T, false>), grid, dim3(kBlock), 0, h->stream, M, N,
-                         alpha, A, lda, x, incx, beta, y, incy, cpg, static_cast<T*>(ws), ctr);
+                         alpha, A, lda, x, incx, beta, y, incy, cpg, part);
+    if (chunks > 1)
+      hipLaunchKernelGGL(gemv_fold_kernel<T>, dim3(unsigned(pb::ceil_div(N, kBlock))),
+                         dim3(kBlock), 0, h->stream, N, int(chunks), part, alpha, beta, y, incy);
   }
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;

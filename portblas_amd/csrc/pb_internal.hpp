@@ -125,5 +125,17 @@ __device__ __forceinline__ int64_t vidx(int64_t i, int64_t n, int64_t inc) {
   return inc >= 0 ? i * inc : (n - 1 - i) * (-inc);
 }
 
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+typedef double f64x2_t __attribute__((ext_vector_type(2)));
+
+// Streaming (non-temporal) load: data read once does not allocate in the
+// caches, so it neither evicts nor forces write-back of other lines.  On
+// MI355X this is worth ~2x of read bandwidth when the caches hold dirty data
+// from earlier kernels.
+template <typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+  return __builtin_nontemporal_load(p);
+}
+
 }  // namespace pbdev
 #endif

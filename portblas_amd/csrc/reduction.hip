@@ -13,14 +13,14 @@
 // r + ld * c whenever rows divides ld (every case its tests use).  This
 // implementation uses the BLAS address for every ld >= rows.
 //
-// MI355X design (HBM-bound, one launch):
+// MI355X design (HBM-bound, non-temporal loads):
 //   INNER: each wave reduces whole columns (lanes along the contiguous rows,
 //          4 loads in flight per lane) and folds across the wave with shuffles;
-//          very tall matrices split the rows into chunks whose partials the
-//          last chunk to finish folds in a fixed order.
+//          very tall matrices split the rows into chunks whose partials a
+//          fold kernel combines in a fixed order.
 //   OUTER: each workgroup owns 256 rows (one per lane, 4 waves split the
-//          columns), folds the waves in LDS; column chunks are folded by the
-//          last chunk to finish.
+//          columns), folds the waves in LDS; column chunks are combined by
+//          the fold kernel.
 #include "pb_internal.hpp"
 
 #include <limits>
@@ -72,26 +72,11 @@ template <typename T> struct Op<T, PB_OP_ABS_ADD> {
   static __device__ T finish(T a, int64_t) { return a; }
 };
 
-__device__ __forceinline__ bool arrive_last(unsigned int* counter, unsigned int total) {
-  __shared__ bool last;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    __threadfence();
-    unsigned int prev = atomicAdd(counter, 1u);
-    last = (prev == total - 1);
-    if (last) *counter = 0u;
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
 // INNER: grid (col_groups, row_chunks); out[c] for c in [0, cols).
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void reduce_inner_kernel(
     int64_t rows, int64_t cols, const T* __restrict__ in, int64_t ld, T* out,
-    int64_t cols_per_group, int64_t rows_per_chunk, T* __restrict__ partials,
-    unsigned int* counters) {
+    int64_t cols_per_group, int64_t rows_per_chunk, T* __restrict__ partials) {
   using O = Op<T, OP>;
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
@@ -106,13 +91,14 @@ __global__ __launch_bounds__(kBlock) void reduce_inner_kernel(
     T a0 = O::init(), a1 = O::init(), a2 = O::init(), a3 = O::init();
     int64_t r = r0 + lane;
     for (; r + 192 < r1; r += 256) {
-      const T x0 = col[r], x1 = col[r + 64], x2 = col[r + 128], x3 = col[r + 192];
+      const T x0 = pbdev::ld_stream(col + r), x1 = pbdev::ld_stream(col + r + 64);
+      const T x2 = pbdev::ld_stream(col + r + 128), x3 = pbdev::ld_stream(col + r + 192);
       a0 = O::apply(a0, x0);
       a1 = O::apply(a1, x1);
       a2 = O::apply(a2, x2);
       a3 = O::apply(a3, x3);
     }
-    for (; r < r1; r += 64) a0 = O::apply(a0, col[r]);
+    for (; r < r1; r += 64) a0 = O::apply(a0, pbdev::ld_stream(col + r));
     T a = O::combine(O::combine(a0, a1), O::combine(a2, a3));
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) a = O::combine(a, __shfl_xor(a, off, 64));
@@ -121,20 +107,27 @@ __global__ __launch_bounds__(kBlock) void reduce_inner_kernel(
       else partials[int64_t(blockIdx.y) * cols + c] = a;
     }
   }
-  if (gridDim.y == 1) return;
-  if (!arrive_last(counters + blockIdx.x, gridDim.y)) return;
-  for (int64_t c = c0 + threadIdx.x; c < c1; c += kBlock) {
-    T t = partials[c];
-    for (unsigned k = 1; k < gridDim.y; ++k) t = O::combine(t, partials[int64_t(k) * cols + c]);
-    out[c] = O::finish(t, rows);
-  }
+}
+
+// Folds the per-chunk partials in a fixed order (a separate launch: stream
+// order publishes the partials without device-scope fences).
+template <typename T, int OP>
+__global__ __launch_bounds__(kBlock) void reduce_fold_kernel(int64_t len, int chunks,
+                                                             const T* __restrict__ partials,
+                                                             T* out, int64_t count) {
+  using O = Op<T, OP>;
+  const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
+  if (i >= len) return;
+  T t = partials[i];
+  for (int k = 1; k < chunks; ++k) t = O::combine(t, partials[int64_t(k) * len + i]);
+  out[i] = O::finish(t, count);
 }
 
 // OUTER: grid (row_tiles, col_chunks); out[r] for r in [0, rows).
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void reduce_outer_kernel(
     int64_t rows, int64_t cols, const T* __restrict__ in, int64_t ld, T* out,
-    int64_t cols_per_chunk, T* __restrict__ partials, unsigned int* counters) {
+    int64_t cols_per_chunk, T* __restrict__ partials) {
   using O = Op<T, OP>;
   __shared__ T red[kWaves][64 * 4 + 1];
   const int lane = threadIdx.x & 63;
@@ -149,7 +142,7 @@ __global__ __launch_bounds__(kBlock) void reduce_outer_kernel(
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
       const int64_t r = row0 + lane + 64 * q;
-      if (r < rows) acc[q] = O::apply(acc[q], col[r]);
+      if (r < rows) acc[q] = O::apply(acc[q], pbdev::ld_stream(col + r));
     }
   }
 #pragma unroll
@@ -165,19 +158,12 @@ __global__ __launch_bounds__(kBlock) void reduce_outer_kernel(
     return;
   }
   if (r < rows) partials[int64_t(blockIdx.y) * rows + r] = s;
-  if (!arrive_last(counters + blockIdx.x, gridDim.y)) return;
-  if (r < rows) {
-    T t = partials[r];
-    for (unsigned k = 1; k < gridDim.y; ++k) t = O::combine(t, partials[int64_t(k) * rows + r]);
-    out[r] = O::finish(t, cols);
-  }
 }
 
 template <typename T, int OP>
 pb_status_t launch_op(pb_handle_t h, pb_reduction_dim_t dim, const T* in, int64_t ld,
                       T* out, int64_t rows, int64_t cols) {
-  const int target = 4 * h->num_cus;
-  unsigned int* ctr = pb::counters(h, 0);
+  const int target = 8 * h->num_cus;
   if (dim == PB_REDUCE_INNER) {
     // Chunks of at least 4096 rows; only split rows when columns are few.
     int64_t groups = pb::ceil_div(cols, kWaves);
@@ -190,16 +176,18 @@ pb_status_t launch_op(pb_handle_t h, pb_reduction_dim_t dim, const T* in, int64_
     } else {
       groups = target * 4 < groups ? target * 4 : groups;
     }
-    if (chunks > 1 && groups > pb::kCounterSlots) groups = pb::kCounterSlots;
     const int64_t cpg = pb::ceil_div(cols, groups);
     groups = pb::ceil_div(cols, cpg);
     const int64_t rpc = pb::ceil_div(rows, chunks);
     chunks = pb::ceil_div(rows, rpc);
     void* ws = nullptr;
     if (chunks > 1) PB_TRY(pb::workspace(h, sizeof(T) * size_t(chunks * cols), &ws));
+    T* part = static_cast<T*>(ws);
     hipLaunchKernelGGL((reduce_inner_kernel<T, OP>), dim3(unsigned(groups), unsigned(chunks)),
-                       dim3(kBlock), 0, h->stream, rows, cols, in, ld, out, cpg, rpc,
-                       static_cast<T*>(ws), ctr);
+                       dim3(kBlock), 0, h->stream, rows, cols, in, ld, out, cpg, rpc, part);
+    if (chunks > 1)
+      hipLaunchKernelGGL((reduce_fold_kernel<T, OP>), dim3(unsigned(pb::ceil_div(cols, kBlock))),
+                         dim3(kBlock), 0, h->stream, cols, int(chunks), part, out, rows);
   } else {
     const int64_t tiles = pb::ceil_div(rows, 256);
     PB_REQUIRE(tiles <= 2147483647LL, "rows too large");
@@ -207,15 +195,17 @@ pb_status_t launch_op(pb_handle_t h, pb_reduction_dim_t dim, const T* in, int64_
     const int64_t max_chunks = pb::ceil_div(cols, 16);
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
-    if (tiles > pb::kCounterSlots) chunks = 1;
     if (chunks > 65535) chunks = 65535;
     const int64_t cpc = pb::ceil_div(cols, chunks);
     chunks = pb::ceil_div(cols, cpc);
     void* ws = nullptr;
     if (chunks > 1) PB_TRY(pb::workspace(h, sizeof(T) * size_t(chunks * rows), &ws));
+    T* part = static_cast<T*>(ws);
     hipLaunchKernelGGL((reduce_outer_kernel<T, OP>), dim3(unsigned(tiles), unsigned(chunks)),
-                       dim3(kBlock), 0, h->stream, rows, cols, in, ld, out, cpc,
-                       static_cast<T*>(ws), ctr);
+                       dim3(kBlock), 0, h->stream, rows, cols, in, ld, out, cpc, part);
+    if (chunks > 1)
+      hipLaunchKernelGGL((reduce_fold_kernel<T, OP>), dim3(unsigned(pb::ceil_div(rows, kBlock))),
+                         dim3(kBlock), 0, h->stream, rows, int(chunks), part, out, cols);
   }
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
