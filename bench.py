@@ -218,8 +218,12 @@ def main():
         log(f"sgemm 1024^3 (cfg1): {res['sgemm_1024_tflops']:.1f} TFLOP/s")
 
         # ---------------- bandwidth-bound: GEMV and BLAS-1 ----------------
-        flushbuf = torch.empty(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
-        flush = lambda: flushbuf.fill_(1.0)
+        # Cold caches before every timed call: read (not write) 1 GiB, so the
+        # L2s and the 256 MB last-level cache hold only clean lines that the
+        # timed kernel can drop without write-back.
+        flushbuf = torch.ones(1 << 28, dtype=torch.float32, device="cuda")  # 1 GiB
+        flush_out = torch.zeros(1, device="cuda")
+        flush = lambda: torch.sum(flushbuf, dim=0, out=flush_out)
         M = 8192
         Ag = torch.rand(M * M, device="cuda") * 2 - 1
         x = torch.rand(M, device="cuda") * 2 - 1

@@ -66,7 +66,8 @@ typedef enum pb_gemm_precision_t {
   PB_GEMM_FP32_SIMT = 1, /* plain FP32 on vector FMA, LDS-tiled */
   PB_GEMM_BF16X3 = 2,    /* split precision: hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulate */
   PB_GEMM_BF16 = 3,      /* inputs rounded to BF16, BF16 MFMA, FP32 accumulate */
-  PB_GEMM_FP8_E4M3 = 4   /* inputs scaled by per-tensor amax to OCP e4m3, FP8 MFMA, FP32 accumulate */
+  PB_GEMM_FP8_E4M3 = 4,  /* inputs scaled by per-tensor amax to OCP e4m3, FP8 MFMA, FP32 accumulate */
+  PB_GEMM_BF16X4 = 5     /* BF16X3 plus the lo*lo term: smaller representation error */
 } pb_gemm_precision_t;
 
 typedef struct pb_handle_s* pb_handle_t;

@@ -141,8 +141,21 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void reduce_fold_kernel(const T* __restrict__ partials,
                                                              int nparts, T* rs) {
+  // All of a thread's loads are issued before the sums (partials come from
+  // other XCDs' L2 write-back: a dependent chain would pay the latency per
+  // element).
+  constexpr int U = 8;
   T s = T(0);
-  for (int i = threadIdx.x; i < nparts; i += kBlock) s += partials[i];
+  for (int base = threadIdx.x; base < nparts; base += kBlock * U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int i = base + u * kBlock;
+      v[u] = i < nparts ? partials[i] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s += v[u];
+  }
   const T total = block_sum(s);
   if (threadIdx.x == 0) {
     if constexpr (OP == kNrm2) rs[0] = sqrt(rs[0] + total);
@@ -248,8 +261,20 @@ __global__ __launch_bounds__(kBlock) void iamax_fold_kernel(const T* __restrict_
                                                             int nparts, IndexT* rs) {
   T bv = init_val<IS_MAX, T>();
   int64_t bi = kSentinel;
-  for (int w = threadIdx.x; w < nparts; w += kBlock)
-    if (better<IS_MAX>(pv[w], pi[w], bv, bi)) { bv = pv[w]; bi = pi[w]; }
+  constexpr int U = 8;
+  for (int base = threadIdx.x; base < nparts; base += kBlock * U) {
+    T v[U];
+    int64_t ix[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int w = base + u * kBlock;
+      v[u] = w < nparts ? pv[w] : init_val<IS_MAX, T>();
+      ix[u] = w < nparts ? pi[w] : kSentinel;
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (better<IS_MAX>(v[u], ix[u], bv, bi)) { bv = v[u]; bi = ix[u]; }
+  }
   block_best<IS_MAX>(bv, bi);
   if (threadIdx.x == 0)
     rs[0] = (bi == kSentinel) ? std::numeric_limits<IndexT>::max() : IndexT(bi);
@@ -263,7 +288,7 @@ inline bool aligned16(const void* p) { return (reinterpret_cast<uintptr_t>(p) & 
 int grid_for(pb_handle_t h, int64_t n, int elems_per_thread_iter) {
   const int64_t per_block = int64_t(kBlock) * elems_per_thread_iter;
   int64_t g = pb::ceil_div(n, per_block);
-  const int64_t cap = int64_t(h->num_cus) * 4;
+  const int64_t cap = int64_t(h->num_cus) * 8;
   if (g > cap) g = cap;
   if (g < 1) g = 1;
   return int(g);

@@ -20,10 +20,10 @@ struct GemmProblem {
   int64_t M, N, K;
   Acc alpha, beta;
   const Acc* alpha_dev;  // optional device factor multiplied into alpha
-  // Up to 3 (A, B) pairs accumulated into one result (split-precision terms).
+  // Up to 4 (A, B) pairs accumulated into one result (split-precision terms).
   int nterms;
-  const Raw* a[3];
-  const Raw* b[3];
+  const Raw* a[4];
+  const Raw* b[4];
   int64_t a_rs, a_ks, a_stride;  // element (m, k) of op(A) at a[m*a_rs + k*a_ks]
   int64_t b_rs, b_ks, b_stride;  // element (n, k) of op(B) at b[n*b_rs + k*b_ks]
   Acc* c;
@@ -136,34 +136,52 @@ void dispatch_layout(pb_handle_t h, const GemmProblem<T>& p, dim3 grid, bool kca
 #undef PB_LAUNCH
 }
 
+// K splits actually used for `splits` requested: every split owns at least
+// one K tile.  Returns the split count and sets the K chunk per split.
+template <int BK>
+inline int clamp_splits(int64_t K, int splits, int64_t& kchunk) {
+  if (splits < 1) splits = 1;
+  const int64_t max_splits = pb::ceil_div(K, BK);
+  if (splits > max_splits) splits = int(max_splits < 1 ? 1 : max_splits);
+  kchunk = pb::ceil_div(pb::ceil_div(K, splits), BK) * BK;
+  if (kchunk < BK) kchunk = BK;
+  splits = int(pb::ceil_div(K, kchunk));
+  return splits < 1 ? 1 : splits;
+}
+
+// Bytes of split-K partial tiles launch_gemm<T, Shape> needs (0 without a split).
+template <typename T, typename Shape>
+size_t splitk_bytes(int64_t M, int64_t N, int64_t K, int64_t batch, int splits) {
+  int64_t kchunk;
+  splits = clamp_splits<Shape::kBK>(K, splits, kchunk);
+  if (splits <= 1) return 0;
+  const int64_t tiles = pb::ceil_div(M, Shape::kBM) * pb::ceil_div(N, Shape::kBN);
+  return sizeof(typename Traits<T>::Acc) * size_t(tiles * batch * splits) * Shape::kBM *
+         Shape::kBN;
+}
+
 // Launches one GEMM problem with tile shape `Shape` and `splits` K splits
 // (clamped so every split owns at least one K tile).  With vector-legal
 // operands the interior tiles run the unguarded kernel and the edge strips a
-// guarded one; otherwise every tile runs guarded.
+// guarded one; otherwise every tile runs guarded.  Split-K partials go to
+// p.partial when the caller provides it (sized by splitk_bytes; callers that
+// keep their own data in the handle workspace must), else to the workspace.
 template <typename T, typename Shape>
 pb_status_t launch_gemm(pb_handle_t h, GemmProblem<T> p, bool kca, bool kcb, bool vec,
                         int splits) {
-  using Acc = typename Traits<T>::Acc;
   constexpr int BM = Shape::kBM, BN = Shape::kBN, BK = Shape::kBK;
   p.tiles_m = pb::ceil_div(p.M, BM);
   p.tiles_n = pb::ceil_div(p.N, BN);
   p.full_m = p.M / BM;
   p.full_n = p.N / BN;
   const int64_t tiles = p.tiles_m * p.tiles_n;
-  if (splits < 1) splits = 1;
-  const int64_t max_splits = pb::ceil_div(p.K, BK);
-  if (splits > max_splits) splits = int(max_splits < 1 ? 1 : max_splits);
-  p.kchunk = pb::ceil_div(pb::ceil_div(p.K, splits), BK) * BK;
-  if (p.kchunk < BK) p.kchunk = BK;
-  splits = int(pb::ceil_div(p.K, p.kchunk));
-  if (splits < 1) splits = 1;
+  splits = clamp_splits<BK>(p.K, splits, p.kchunk);
   p.splits = splits;
-  p.partial = nullptr;
   p.counters = pb::counters(h, 0);
-  if (splits > 1) {
+  if (splits > 1 && p.partial == nullptr) {
     void* ws = nullptr;
-    PB_TRY(pb::workspace(h, sizeof(Acc) * size_t(tiles * p.batch * splits) * BM * BN, &ws));
-    p.partial = static_cast<Acc*>(ws);
+    PB_TRY(pb::workspace(h, splitk_bytes<T, Shape>(p.M, p.N, p.K, p.batch, splits), &ws));
+    p.partial = static_cast<typename Traits<T>::Acc*>(ws);
   }
   const unsigned gy = unsigned(p.batch < 65535 ? p.batch : 65535);
   const int64_t interior = vec ? p.full_m * p.full_n : 0;

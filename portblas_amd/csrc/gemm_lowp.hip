@@ -6,14 +6,16 @@
 // writes it K-contiguous (transposing through LDS when X is M/N-contiguous) in
 // the low-precision format:
 //   BF16x3:  hi = bf16(x), lo = bf16(x - hi); C = A_hi B_lo + A_lo B_hi + A_hi B_hi
+//   BF16x4:  BF16x3 plus the A_lo B_lo term
 //   BF16:    hi only
 //   FP8:     q = e4m3(x * 448 / amax(X)); the epilogue multiplies by
 //            amax(A) amax(B) / 448^2 (read on the device, no host round trip)
-// The GEMM then runs the MFMA tile engine on the packed operands (every
-// operand K-contiguous and 16-byte aligned, so interior tiles take the
-// unguarded fast path).
+// The GEMM then runs on the packed operands (every operand K-contiguous and
+// 16-byte aligned): the 256 x 256 fused-term kernel (gemm_mt.hpp) when the
+// problem fills the GPU with such tiles, else the MFMA tile engine.
 #include "gemm_front.hpp"
 #include "gemm_launch.hpp"
+#include "gemm_mt.hpp"
 #include "lowp.hpp"
 
 #include <cmath>
@@ -118,6 +120,120 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
   }
 }
 
+// Converts v (already scaled for FP8) to the packed format(s).
+template <int MODE>
+__device__ __forceinline__ void pack_convert(float v, uint16_t& hi, uint16_t& lo, uint8_t& q) {
+  if constexpr (MODE == kPackFp8) {
+    q = pblowp::f32_to_e4m3(v);
+  } else {
+    hi = pblowp::f32_to_bf16(v);
+    if constexpr (MODE == kPackBf16Split) lo = pblowp::f32_to_bf16(v - pblowp::bf16_to_f32(hi));
+  }
+}
+
+// Vectorized pre-pass for 16-byte aligned X with 4-element aligned pitches.
+// Same contract as pack_kernel.  KC (X K-contiguous): every thread converts
+// 4 consecutive k of one row straight from registers.  Otherwise (X
+// row-contiguous): 16-byte loads along the rows into an LDS tile, then every
+// thread writes 8 consecutive k of one row (8 threads cover a row's 64 k).
+template <int MODE, bool KC>
+__global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__ x, int64_t rs,
+                                                       int64_t ks, int64_t rows, int64_t K,
+                                                       int64_t stride, int64_t kp, void* dst_hi,
+                                                       void* dst_lo, const float* scales,
+                                                       int scale_idx) {
+  const int64_t r0 = int64_t(blockIdx.x) * 64;
+  const int64_t k0 = int64_t(blockIdx.y) * 64;
+  const int64_t b = blockIdx.z;
+  const float* xb = x + b * stride;
+  const float s = (MODE == kPackFp8) ? scales[scale_idx] : 1.0f;
+  auto* hi16 = static_cast<uint16_t*>(dst_hi);
+  auto* lo16 = static_cast<uint16_t*>(dst_lo);
+  auto* q8 = static_cast<uint8_t*>(dst_hi);
+  if constexpr (KC) {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int64_t r = r0 + idx / 16, k = k0 + 4 * (idx % 16);
+      if (r >= rows || k >= kp) continue;
+      float v[4];
+      if (k + 3 < K) {
+        const pbdev::f32x4_t t =
+            pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r * rs + k));
+        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? xb[r * rs + k + e] : 0.f;
+      }
+      const int64_t o = (b * rows + r) * kp + k;
+      uint16_t h[4], l[4];
+      uint8_t q[4];
+#pragma unroll
+      for (int e = 0; e < 4; ++e) pack_convert<MODE>(v[e] * s, h[e], l[e], q[e]);
+      if constexpr (MODE == kPackFp8) {
+        uint32_t w;
+        __builtin_memcpy(&w, q, 4);
+        *reinterpret_cast<uint32_t*>(q8 + o) = w;
+      } else {
+        uint2 w;
+        __builtin_memcpy(&w, h, 8);
+        *reinterpret_cast<uint2*>(hi16 + o) = w;
+        if constexpr (MODE == kPackBf16Split) {
+          __builtin_memcpy(&w, l, 8);
+          *reinterpret_cast<uint2*>(lo16 + o) = w;
+        }
+      }
+    }
+  } else {
+    __shared__ float tile[64][65];  // [k][row]
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int kk = idx / 16, rq = 4 * (idx % 16);
+      const int64_t r = r0 + rq, k = k0 + kk;
+      float v[4] = {0.f, 0.f, 0.f, 0.f};
+      if (k < K) {
+        if (r + 3 < rows) {
+          const pbdev::f32x4_t t =
+              pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r + k * ks));
+          v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) v[e] = (r + e < rows) ? xb[r + e + k * ks] : 0.f;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < 4; ++e) tile[kk][rq + e] = v[e];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int item = threadIdx.x + 256 * j;
+      const int kg = item % 8, rr = item / 8;
+      const int64_t r = r0 + rr, k = k0 + 8 * kg;
+      if (r >= rows || k >= kp) continue;
+      const int64_t o = (b * rows + r) * kp + k;
+      uint16_t h[8], l[8];
+      uint8_t q[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) pack_convert<MODE>(tile[8 * kg + e][rr] * s, h[e], l[e], q[e]);
+      if constexpr (MODE == kPackFp8) {
+        uint2 w;
+        __builtin_memcpy(&w, q, 8);
+        *reinterpret_cast<uint2*>(q8 + o) = w;
+      } else {
+        uint4 w;
+        __builtin_memcpy(&w, h, 16);
+        *reinterpret_cast<uint4*>(hi16 + o) = w;
+        if constexpr (MODE == kPackBf16Split) {
+          __builtin_memcpy(&w, l, 16);
+          *reinterpret_cast<uint4*>(lo16 + o) = w;
+        }
+      }
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Tile configurations (the autotuner's search space for the BF16 engine):
 // 8 tile shapes x 5 split-K factors.
@@ -169,6 +285,22 @@ pb_status_t launch_config(pb_handle_t h, int id, const GemmProblem<T>& p, bool k
   return pb::set_error(PB_STATUS_INVALID_VALUE, "config id %d out of range", id);
 }
 
+// Split-K partial bytes config `id` needs for an M x N x K problem (bf16).
+size_t config_splitk_bytes(int id, int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int shape = id / kNumSplits, splits = kSplits[id % kNumSplits];
+  switch (shape) {
+    case 0: return splitk_bytes<bf16_t, B128>(M, N, K, batch, splits);
+    case 1: return splitk_bytes<bf16_t, B128x64>(M, N, K, batch, splits);
+    case 2: return splitk_bytes<bf16_t, B64x128>(M, N, K, batch, splits);
+    case 3: return splitk_bytes<bf16_t, B64>(M, N, K, batch, splits);
+    case 4: return splitk_bytes<bf16_t, B256x128>(M, N, K, batch, splits);
+    case 5: return splitk_bytes<bf16_t, B128x256>(M, N, K, batch, splits);
+    case 6: return splitk_bytes<bf16_t, B128k32>(M, N, K, batch, splits);
+    case 7: return splitk_bytes<bf16_t, B256>(M, N, K, batch, splits);
+  }
+  return 0;
+}
+
 // Built-in rule (used when no learned selector is installed): the largest
 // tile that still gives every CU two workgroups, else split K.
 int default_config(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch) {
@@ -212,16 +344,42 @@ pb_status_t run_lowp(pb_handle_t h, int id, GemmProblem<T> p, bool kca, bool kcb
   return launch_config<T>(h, id, p, kca, kcb, vec);
 }
 
+// The 256 x 256 kernel pays off when its tiles fill every CU at least once
+// and little of them is padding.
+bool mt_eligible(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch) {
+  if (batch > 65535 || K <= 0) return false;
+  const int64_t tiles = pb::ceil_div(M, mt::kBM) * pb::ceil_div(N, mt::kBN) * batch;
+  const double waste = double(tiles) * mt::kBM * mt::kBN / (double(M) * double(N) * double(batch));
+  return tiles >= h->num_cus && waste < 1.15 && mt::fits(M, N, pb::align_up(size_t(K), 64));
+}
+
 pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall<float>& g) {
   const int64_t M = g.M, N = g.N, K = g.K, batch = g.batch;
   const bool fp8 = prec == PB_GEMM_FP8_E4M3;
-  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1), fp8 ? 16 : 8);
+  const bool split = prec == PB_GEMM_BF16X3 || prec == PB_GEMM_BF16X4;
+  const bool use_mt = !fp8 && mt_eligible(h, M, N, K, batch);
+  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1), use_mt ? 64 : (fp8 ? 16 : 8));
   const size_t esz = fp8 ? 1 : 2;
-  const int nparts = prec == PB_GEMM_BF16X3 ? 2 : 1;
+  const int nparts = split ? 2 : 1;
   const size_t abytes = pb::align_up(size_t(M) * kp * esz * batch, 256);
   const size_t bbytes = pb::align_up(size_t(N) * kp * esz * batch, 256);
+  // The GEMM config (and so its split-K partials) is fixed before the
+  // workspace is taken: packed operands and partials share one allocation.
+  int id = -1, fp8_splits = 1;
+  size_t part_bytes = 0;
+  if (fp8) {
+    const int64_t t = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
+    while (t * fp8_splits < 2 * h->num_cus && K / (fp8_splits * 2) >= 512 && fp8_splits < 16)
+      fp8_splits *= 2;
+    part_bytes = splitk_bytes<fp8_t, Q128>(M, N, K, batch, fp8_splits);
+  } else if (!use_mt) {
+    id = selector_choose(h, 'n', 'n', M, N, K, batch);
+    if (id < 0) id = default_config(h, M, N, K, batch);
+    PB_REQUIRE(id >= 0 && id < kNumConfigs, "config id %d out of range", id);
+    part_bytes = config_splitk_bytes(id, M, N, K, batch);
+  }
   void* ws = nullptr;
-  PB_TRY(pb::workspace(h, nparts * (abytes + bbytes) + 1024, &ws));
+  PB_TRY(pb::workspace(h, nparts * (abytes + bbytes) + part_bytes + 1024, &ws));
   Scratch sc{static_cast<uint8_t*>(ws)};
   void* a_hi = sc.take<uint8_t>(abytes);
   void* a_lo = nparts == 2 ? sc.take<uint8_t>(abytes) : nullptr;
@@ -229,6 +387,7 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   void* b_lo = nparts == 2 ? sc.take<uint8_t>(bbytes) : nullptr;
   float* scales = sc.take<float>(4);
   unsigned int* amax_bits = sc.take<unsigned int>(2);
+  float* partial = part_bytes ? sc.take<float>(part_bytes / sizeof(float)) : nullptr;
 
   // Operand strides of op(A) (rows M) and op(B) (rows N), as for the engine.
   const int64_t a_rs = g.ta == 'n' ? 1 : g.lda, a_ks = g.ta == 'n' ? g.lda : 1;
@@ -249,7 +408,29 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   auto pack = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, void* hi,
                   void* lo, int sidx) {
     dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(kp, 64)), unsigned(batch)};
-    if (prec == PB_GEMM_BF16X3)
+    // 16-byte loads need an aligned base and 4-element pitches; the packed
+    // pitch kp is a multiple of 8.
+    const bool kc = ks == 1;
+    const int64_t pitch = kc ? rs : ks;
+    const bool vec = aligned16(x) && pitch % 4 == 0 && (batch == 1 || stride % 4 == 0) &&
+                     (rs == 1 || ks == 1);
+    if (vec) {
+#define PB_PACKV(MODE)                                                                      \
+  do {                                                                                      \
+    if (kc)                                                                                 \
+      hipLaunchKernelGGL((pack_vec_kernel<MODE, true>), grid, dim3(256), 0, h->stream, x, rs, \
+                         ks, rows, K, stride, kp, hi, lo, scales, sidx);                    \
+    else                                                                                    \
+      hipLaunchKernelGGL((pack_vec_kernel<MODE, false>), grid, dim3(256), 0, h->stream, x,  \
+                         rs, ks, rows, K, stride, kp, hi, lo, scales, sidx);                \
+  } while (0)
+      if (split) PB_PACKV(kPackBf16Split);
+      else if (prec == PB_GEMM_BF16) PB_PACKV(kPackBf16);
+      else PB_PACKV(kPackFp8);
+#undef PB_PACKV
+      return;
+    }
+    if (split)
       hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
                          rows, K, stride, kp, hi, lo, scales, sidx);
     else if (prec == PB_GEMM_BF16)
@@ -274,23 +455,35 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
     p.a_rs = kp; p.a_ks = 1; p.a_stride = M * kp;
     p.b_rs = kp; p.b_ks = 1; p.b_stride = N * kp;
     p.c = g.c; p.c_rs = 1; p.ldc = g.ldc; p.c_stride = g.sc; p.batch = batch; p.c_vec = c_vec;
-    const int64_t t = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
-    int splits = 1;
-    while (t * splits < 2 * h->num_cus && K / (splits * 2) >= 512 && splits < 16) splits *= 2;
-    return launch_gemm<fp8_t, Q128>(h, p, true, true, true, splits);
+    p.partial = partial;
+    return launch_gemm<fp8_t, Q128>(h, p, true, true, true, fp8_splits);
   }
-  GemmProblem<bf16_t> p{};
-  p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;
   const auto* ah = static_cast<const uint16_t*>(a_hi);
   const auto* al = static_cast<const uint16_t*>(a_lo);
   const auto* bh = static_cast<const uint16_t*>(b_hi);
   const auto* bl = static_cast<const uint16_t*>(b_lo);
-  if (prec == PB_GEMM_BF16X3) {
+  if (use_mt) {
+    mt::Problem q{};
+    q.M = int(M); q.N = int(N); q.kp = int(kp);
+    q.alpha = g.alpha; q.beta = g.beta; q.alpha_dev = nullptr;
+    q.a[0] = ah; q.a[1] = al; q.b[0] = bh; q.b[1] = bl;
+    q.a_stride = M * kp; q.b_stride = N * kp;
+    q.c = g.c; q.ldc = g.ldc; q.c_stride = g.sc; q.batch = int(batch); q.c_vec = c_vec;
+    if (prec == PB_GEMM_BF16X3) return mt::launch<32, 3>(h, q);
+    if (prec == PB_GEMM_BF16X4) return mt::launch<32, 4>(h, q);
+    return mt::launch<64, 1>(h, q);
+  }
+  GemmProblem<bf16_t> p{};
+  p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;
+  if (split) {
     // Small terms first, then the leading term.
-    p.nterms = 3;
+    p.nterms = prec == PB_GEMM_BF16X4 ? 4 : 3;
     p.a[0] = ah; p.b[0] = bl;
     p.a[1] = al; p.b[1] = bh;
-    p.a[2] = ah; p.b[2] = bh;
+    if (prec == PB_GEMM_BF16X4) {
+      p.a[2] = al; p.b[2] = bl;
+    }
+    p.a[p.nterms - 1] = ah; p.b[p.nterms - 1] = bh;
   } else {
     p.nterms = 1;
     p.a[0] = ah; p.b[0] = bh;
@@ -298,7 +491,8 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   p.a_rs = kp; p.a_ks = 1; p.a_stride = M * kp;
   p.b_rs = kp; p.b_ks = 1; p.b_stride = N * kp;
   p.c = g.c; p.c_rs = 1; p.ldc = g.ldc; p.c_stride = g.sc; p.batch = batch; p.c_vec = c_vec;
-  return run_lowp<bf16_t>(h, -1, p, true, true, true);
+  p.partial = partial;
+  return launch_config<bf16_t>(h, id, p, true, true, true);
 }
 
 // BF16-input GEMM on the caller's layout (no pre-pass).
@@ -351,7 +545,8 @@ pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t prec, char ta, char tb
                        prec == PB_GEMM_FP32_SIMT);
   pb::clear_error();
   PB_TRY(pb::enter(h));
-  if (prec != PB_GEMM_BF16X3 && prec != PB_GEMM_BF16 && prec != PB_GEMM_FP8_E4M3)
+  if (prec != PB_GEMM_BF16X3 && prec != PB_GEMM_BF16X4 && prec != PB_GEMM_BF16 &&
+      prec != PB_GEMM_FP8_E4M3)
     return pb::set_error(PB_STATUS_INVALID_VALUE, "invalid precision %d", int(prec));
   GemmCall<float> g;
   bool done = false;
@@ -399,6 +594,17 @@ pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k
   PB_REQUIRE(k == 0 || (a_hi && a_lo && b_hi && b_lo), "null operand");
   PB_REQUIRE(ldpa >= k && ldpb >= k, "packed pitch smaller than k");
   if (alpha == 0.f) return sgemm_alpha_zero(h, m, n, beta, c, ldc, 0, 1);
+  const bool aligned = aligned16(a_hi) && aligned16(a_lo) && aligned16(b_hi) && aligned16(b_lo);
+  if (ldpa == ldpb && ldpa % 32 == 0 && aligned && mt_eligible(h, m, n, k, 1) &&
+      mt::fits(m, n, ldpa)) {
+    // Columns [k, ldp) of the packed rows are zero (pb_pack_bf16x3 writes them).
+    mt::Problem q{};
+    q.M = int(m); q.N = int(n); q.kp = int(ldpa);
+    q.alpha = alpha; q.beta = beta; q.alpha_dev = nullptr;
+    q.a[0] = a_hi; q.a[1] = a_lo; q.b[0] = b_hi; q.b[1] = b_lo;
+    q.c = c; q.ldc = ldc; q.batch = 1; q.c_vec = aligned16(c) && ldc % 4 == 0;
+    return mt::launch<32, 3>(h, q);
+  }
   GemmProblem<bf16_t> p{};
   p.M = m; p.N = n; p.K = k; p.alpha = alpha; p.beta = beta;
   p.nterms = 3;

@@ -7,8 +7,8 @@
 // y = beta*y + alpha*sum (beta == 0 -> y not read, :185-212).
 //
 // MI355X design (HBM-bound; A is read once with non-temporal 16-byte loads):
-//  'n'  a workgroup of 4 waves owns 256 consecutive rows; every lane holds 4
-//       consecutive rows (one 16-byte load per column for float) and the 4
+//  'n'  a workgroup of 8 waves owns 256 consecutive rows; every lane holds 4
+//       consecutive rows (one 16-byte load per column for float) and the 8
 //       waves split the workgroup's column chunk.  Partial sums are folded
 //       across waves in LDS; with several column chunks, each writes its
 //       partials to the workspace and a fold kernel sums them in a fixed
@@ -50,86 +50,72 @@ __device__ __forceinline__ void store_y(T* y, int64_t idx, T alpha, T beta, T su
 }
 
 // ---------------------------------------------------------------------------
-// 'n': grid (row_chunks, col_chunks).  A workgroup owns kTRows (2048) rows;
-// every wave walks its columns (the 4 waves interleave them) and reads each
-// column's 2048-row segment as one contiguous 8 KB stream, lane rows
-// r0 + 256 q + 4 lane + e (VEC: A 16-byte aligned with lda a multiple of 4
-// elements) or r0 + 64 (4 q + e) + lane (scalar).  The 4 waves' sums are
-// folded in LDS in a fixed order.
+// 'n': grid (row_tiles, col_chunks), kNBlock threads (kNWaves waves).  A
+// workgroup owns kNRows (256) consecutive rows; every lane holds 4 of them
+// (rows r0 + 4 lane + e: one 16-byte load per column for float, VEC = A
+// 16-byte aligned with lda a multiple of 4 elements; scalar: r0 + 64 e +
+// lane).  The waves interleave the chunk's columns, kNU columns per step, so
+// each wave keeps kNU 1 KB column segments in flight.  The waves' sums are
+// folded in LDS in a fixed order.  Eight waves per workgroup keep the number
+// of column chunks (and so the fold traffic) small.
 // ---------------------------------------------------------------------------
+constexpr int kNWaves = 8;
+constexpr int kNBlock = 64 * kNWaves;
+constexpr int kNRows = 256;
+constexpr int kNU = 8;
+
 template <typename T, bool VEC>
-__global__ __launch_bounds__(kBlock) void gemv_n_kernel(
+__global__ __launch_bounds__(kNBlock) void gemv_n_kernel(
     int64_t M, int64_t N, T alpha, const T* __restrict__ A, int64_t lda,
     const T* __restrict__ x, int64_t incx, T beta, T* y, int64_t incy,
     int64_t cols_per_chunk, T* __restrict__ partials) {
-  __shared__ T red[kWaves][kTRows];
+  __shared__ T red[kNWaves][kNRows];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t r0 = int64_t(blockIdx.x) * kTRows;
+  const int64_t r0 = int64_t(blockIdx.x) * kNRows;
   const int64_t c0 = int64_t(blockIdx.y) * cols_per_chunk;
   int64_t c1 = c0 + cols_per_chunk;
   if (c1 > N) c1 = N;
-  T acc[kTQ][4];
-#pragma unroll
-  for (int q = 0; q < kTQ; ++q)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) acc[q][e] = T(0);
-  const bool full = r0 + kTRows <= M;
-  if (VEC && full) {
+  T acc[4] = {T(0), T(0), T(0), T(0)};
+  if (VEC && r0 + kNRows <= M) {
     const T* Ap = A + r0 + 4 * lane;
     int64_t j = c0 + wave;
-    // Two columns per step: 2 x kTQ 16-byte loads in flight per lane.
-    for (; j + kWaves < c1; j += 2 * kWaves) {
-      Four<T> a[kTQ], b[kTQ];
+    for (; j + (kNU - 1) * kNWaves < c1; j += kNU * kNWaves) {
+      Four<T> a[kNU];
+      T xs[kNU];
 #pragma unroll
-      for (int q = 0; q < kTQ; ++q) a[q] = load4(Ap + j * lda + 256 * q);
+      for (int u = 0; u < kNU; ++u) a[u] = load4(Ap + (j + u * kNWaves) * lda);
 #pragma unroll
-      for (int q = 0; q < kTQ; ++q) b[q] = load4(Ap + (j + kWaves) * lda + 256 * q);
-      const T xa = x[pbdev::vidx(j, N, incx)];
-      const T xb = x[pbdev::vidx(j + kWaves, N, incx)];
+      for (int u = 0; u < kNU; ++u) xs[u] = x[pbdev::vidx(j + u * kNWaves, N, incx)];
 #pragma unroll
-      for (int q = 0; q < kTQ; ++q)
+      for (int u = 0; u < kNU; ++u)
 #pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          acc[q][e] += a[q].v[e] * xa;
-          acc[q][e] += b[q].v[e] * xb;
-        }
+        for (int e = 0; e < 4; ++e) acc[e] += a[u].v[e] * xs[u];
     }
-    for (; j < c1; j += kWaves) {
-      const T xa = x[pbdev::vidx(j, N, incx)];
+    for (; j < c1; j += kNWaves) {
+      const Four<T> a = load4(Ap + j * lda);
+      const T xj = x[pbdev::vidx(j, N, incx)];
 #pragma unroll
-      for (int q = 0; q < kTQ; ++q) {
-        const Four<T> a = load4(Ap + j * lda + 256 * q);
-#pragma unroll
-        for (int e = 0; e < 4; ++e) acc[q][e] += a.v[e] * xa;
-      }
+      for (int e = 0; e < 4; ++e) acc[e] += a.v[e] * xj;
     }
   } else {
-    for (int64_t j = c0 + wave; j < c1; j += kWaves) {
+    for (int64_t j = c0 + wave; j < c1; j += kNWaves) {
       const T xj = x[pbdev::vidx(j, N, incx)];
       const T* col = A + j * lda;
 #pragma unroll
-      for (int q = 0; q < kTQ; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          const int64_t r = VEC ? r0 + 256 * q + 4 * lane + e : r0 + 64 * (4 * q + e) + lane;
-          if (r < M) acc[q][e] += pbdev::ld_stream(col + r) * xj;
-        }
+      for (int e = 0; e < 4; ++e) {
+        const int64_t r = VEC ? r0 + 4 * lane + e : r0 + 64 * e + lane;
+        if (r < M) acc[e] += pbdev::ld_stream(col + r) * xj;
+      }
     }
   }
-  // Fold the 4 waves in LDS (fixed order), one row per thread per pass.
 #pragma unroll
-  for (int q = 0; q < kTQ; ++q)
-#pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      const int rr = VEC ? 256 * q + 4 * lane + e : 64 * (4 * q + e) + lane;
-      red[wave][rr] = acc[q][e];
-    }
+  for (int e = 0; e < 4; ++e) red[wave][VEC ? 4 * lane + e : 64 * e + lane] = acc[e];
   __syncthreads();
-  for (int rr = threadIdx.x; rr < kTRows; rr += kBlock) {
+  for (int rr = threadIdx.x; rr < kNRows; rr += kNBlock) {
     T s = red[0][rr];
 #pragma unroll
-    for (int w = 1; w < kWaves; ++w) s += red[w][rr];
+    for (int w = 1; w < kNWaves; ++w) s += red[w][rr];
     const int64_t r = r0 + rr;
     if (r >= M) continue;
     if (gridDim.y == 1) store_y(y, pbdev::vidx(r, M, incy), alpha, beta, s);
@@ -146,8 +132,15 @@ __global__ __launch_bounds__(kBlock) void gemv_fold_kernel(int64_t len, int chun
                                                            T alpha, T beta, T* y, int64_t incy) {
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= len) return;
+  constexpr int U = 8;  // loads in flight before the (fixed-order) sums
   T t = T(0);
-  for (int c = 0; c < chunks; ++c) t += partials[int64_t(c) * len + i];
+  for (int c0 = 0; c0 < chunks; c0 += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = c0 + u < chunks ? partials[int64_t(c0 + u) * len + i] : T(0);
+#pragma unroll
+    for (int u = 0; u < U; ++u) t += v[u];
+  }
   store_y(y, pbdev::vidx(i, len, incy), alpha, beta, t);
 }
 
@@ -184,7 +177,37 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
       }
     }
   }
-  for (int64_t j = j0 + wave; j < j1; j += kWaves) {
+  auto emit = [&](int64_t j, T s) {
+    s = pbdev::wave_sum(s);
+    if (lane == 0) {
+      if (gridDim.y == 1) store_y(y, pbdev::vidx(j, N, incy), alpha, beta, s);
+      else partials[int64_t(blockIdx.y) * N + j] = s;
+    }
+  };
+  int64_t j = j0 + wave;
+  if (VEC && full) {
+    // Two columns per step: 2 x kTQ 16-byte loads in flight per lane.
+    for (; j + kWaves < j1; j += 2 * kWaves) {
+      const T* ca = A + j * lda + r0 + 4 * lane;
+      const T* cb = ca + kWaves * lda;
+      Four<T> a[kTQ], b[kTQ];
+#pragma unroll
+      for (int q = 0; q < kTQ; ++q) a[q] = load4(ca + 256 * q);
+#pragma unroll
+      for (int q = 0; q < kTQ; ++q) b[q] = load4(cb + 256 * q);
+      T sa = T(0), sb = T(0);
+#pragma unroll
+      for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          sa += a[q].v[e] * xr[q][e];
+          sb += b[q].v[e] * xr[q][e];
+        }
+      emit(j, sa);
+      emit(j + kWaves, sb);
+    }
+  }
+  for (; j < j1; j += kWaves) {
     const T* col = A + j * lda;
     T acc = T(0);
     if (VEC && full) {
@@ -204,11 +227,7 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
           if (r < M) acc += pbdev::ld_stream(col + r) * xr[q][e];
         }
     }
-    acc = pbdev::wave_sum(acc);
-    if (lane == 0) {
-      if (gridDim.y == 1) store_y(y, pbdev::vidx(j, N, incy), alpha, beta, acc);
-      else partials[int64_t(blockIdx.y) * N + j] = acc;
-    }
+    emit(j, acc);
   }
 }
 
@@ -252,10 +271,11 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
   const bool avec = aligned16(A) && (lda * int64_t(sizeof(T))) % 16 == 0;
   const int target = 4 * h->num_cus;
   if (!tr) {
-    const int64_t tiles = pb::ceil_div(M, kTRows);
+    // About two 8-wave workgroups per CU; every chunk keeps >= 8 columns per wave.
+    const int64_t tiles = pb::ceil_div(M, kNRows);
     PB_REQUIRE(tiles <= 0x7fffffffLL, "M too large");
-    int64_t chunks = pb::ceil_div(target, tiles);
-    const int64_t max_chunks = pb::ceil_div(N, 4 * kWaves);  // >= 16 columns per chunk
+    int64_t chunks = pb::ceil_div(2 * h->num_cus, tiles);
+    const int64_t max_chunks = pb::ceil_div(N, 8 * kNWaves);
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
     if (chunks > 65535) chunks = 65535;
@@ -266,10 +286,10 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
     T* part = static_cast<T*>(ws);
     dim3 grid{unsigned(tiles), unsigned(chunks)};
     if (avec)
-      hipLaunchKernelGGL((gemv_n_kernel<T, true>), grid, dim3(kBlock), 0, h->stream, M, N,
+      hipLaunchKernelGGL((gemv_n_kernel<T, true>), grid, dim3(kNBlock), 0, h->stream, M, N,
                          alpha, A, lda, x, incx, beta, y, incy, cpc, part);
     else
-      hipLaunchKernelGGL((gemv_n_kernel<T, false>), grid, dim3(kBlock), 0, h->stream, M, N,
+      hipLaunchKernelGGL((gemv_n_kernel<T, false>), grid, dim3(kNBlock), 0, h->stream, M, N,
                          alpha, A, lda, x, incx, beta, y, incy, cpc, part);
     if (chunks > 1)
       hipLaunchKernelGGL(gemv_fold_kernel<T>, dim3(unsigned(pb::ceil_div(M, kBlock))),

@@ -118,8 +118,17 @@ __global__ __launch_bounds__(kBlock) void reduce_fold_kernel(int64_t len, int ch
   using O = Op<T, OP>;
   const int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x;
   if (i >= len) return;
+  constexpr int U = 8;  // loads in flight before the (fixed-order) combines
   T t = partials[i];
-  for (int k = 1; k < chunks; ++k) t = O::combine(t, partials[int64_t(k) * len + i]);
+  for (int k0 = 1; k0 < chunks; k0 += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u < chunks) v[u] = partials[int64_t(k0 + u) * len + i];
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (k0 + u < chunks) t = O::combine(t, v[u]);
+  }
   out[i] = O::finish(t, count);
 }
 

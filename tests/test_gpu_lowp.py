@@ -1,0 +1,191 @@
+"""Reduced-precision GEMMs (pb_gemm_ex with BF16x3 / BF16x4 / BF16 / FP8-e4m3,
+the pack-once API and the BF16-input GEMM) on the GPU against an FP64
+reference, with forward-error bounds derived from each format's rounding:
+
+  BF16x3  x = hi + lo + r, |x - hi| <= 2^-8 |x|, |r| <= 2^-16 |x|; the three
+          kept products miss hi*r_b + r_a*hi + lo*lo (<= 3.1 * 2^-16 |a||b|)
+  BF16x4  also keeps lo*lo                          (<= 2.1 * 2^-16 |a||b|)
+  BF16    |ab - hi_a hi_b| <= (2 * 2^-8 + 2^-16) |a||b|
+  FP8     e4m3 after scaling by 448 / amax: relative 2^-4 plus an absolute
+          2^-10 * amax / 448 below the normal range
+plus FP32 accumulation, (terms * K + 2) * 2^-24 |a||b|.  Shapes cover the
+256 x 256 fused-term kernel (large M, N, ragged edges, K not a multiple of
+the K step, batches) and the tile engine (small shapes)."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U = 2.0 ** -24
+
+
+def cm(flat, rows, cols, ld):
+    """Column-major rows x cols view of a flat torch buffer."""
+    return flat[: cols * ld].view(cols, ld).t()[:rows, :]
+
+
+def make(torch, n, seed, dist="uniform"):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    if dist == "uniform":
+        return torch.rand(n, generator=g, device="cuda") * 2 - 1
+    if dist == "normal":
+        return torch.randn(n, generator=g, device="cuda")
+    mag = torch.exp2(torch.rand(n, generator=g, device="cuda") * 16 - 8)
+    sign = torch.where(torch.rand(n, generator=g, device="cuda") < 0.5, -1.0, 1.0)
+    return (mag * sign).float()
+
+
+def check_lowp(handle, torch, prec, ta, tb, m, n, k, alpha=1.0, beta=0.0, batch=1, seed=0,
+               dist="uniform"):
+    import portblas_amd as pb
+
+    P = pb.gemm_precision_t
+    ar, ac = (m, k) if ta == "n" else (k, m)
+    br, bc = (k, n) if tb == "n" else (n, k)
+    lda, ldb, ldc = max(ar, 1), max(br, 1), max(m, 1)
+    sa, sb, sc = lda * ac, ldb * bc, ldc * n
+    A = make(torch, sa * batch, seed * 3 + 1, dist)
+    B = make(torch, sb * batch, seed * 3 + 2, dist)
+    C = make(torch, sc * batch, seed * 3 + 3)
+    C0 = C.clone()
+    pb._gemm_ex(handle, prec, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb, sc, batch)
+    torch.cuda.synchronize()
+    terms = {P.bf16x3: 3, P.bf16x4: 4, P.bf16: 1, P.fp8_e4m3: 1}[prec]
+    rep = {P.bf16x3: 3.1 * 2.0 ** -16, P.bf16x4: 2.1 * 2.0 ** -16, P.bf16: 2.01 * 2.0 ** -8,
+           P.fp8_e4m3: 0.13}[prec]
+    amax_a = A.abs().max().double().item()
+    amax_b = B.abs().max().double().item()
+    worst = 0.0
+    for b in range(batch):
+        oa = cm(A[b * sa:], ar, ac, lda).double()
+        ob = cm(B[b * sb:], br, bc, ldb).double()
+        oa = oa if ta == "n" else oa.t()
+        ob = ob if tb == "n" else ob.t()
+        c0 = cm(C0[b * sc:], m, n, ldc).double()
+        ref = alpha * (oa @ ob) + beta * c0
+        absprod = oa.abs() @ ob.abs()
+        tol = (rep + 1.01 * (terms * k + 2) * U) * abs(alpha) * absprod + 2 * U * abs(beta) * c0.abs()
+        if prec == P.fp8_e4m3:
+            ea, eb = 2.0 ** -10 * amax_a / 448, 2.0 ** -10 * amax_b / 448
+            sub = 1.07 * (ea * ob.abs().sum(0, keepdim=True) + eb * oa.abs().sum(1, keepdim=True)) + k * ea * eb
+            tol = tol + abs(alpha) * sub
+        got = cm(C[b * sc:], m, n, ldc).double()
+        assert torch.isfinite(got).all(), "non-finite output"
+        err = (got - ref).abs()
+        r = (err / (tol + 1e-30)).max().item()
+        worst = max(worst, r)
+    assert worst <= 1.0, f"max err/tol {worst:.3g}"
+    return worst
+
+
+PRECS = ["bf16x3", "bf16x4", "bf16", "fp8_e4m3"]
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("ta,tb", [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")])
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (77, 130, 200), (33, 17, 1000), (300, 260, 64)])
+def test_lowp_small(handle, torch_cuda, prec, ta, tb, m, n, k):
+    import portblas_amd as pb
+
+    check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, m, n, k, 1.5, 0.5)
+
+
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16x4", "bf16"])
+@pytest.mark.parametrize("ta,tb,m,n,k,batch", [
+    ("n", "n", 4096, 4096, 256, 1),   # whole 256 x 256 tiles
+    ("t", "n", 4000, 4100, 1000, 1),  # ragged M, N edges; K tail inside the packed pitch
+    ("n", "t", 4100, 4000, 72, 1),
+    ("t", "t", 2048, 2048, 96, 4),    # batched
+])
+def test_lowp_large_tiles(handle, torch_cuda, prec, ta, tb, m, n, k, batch):
+    import portblas_amd as pb
+
+    check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, m, n, k, -1.25, 0.75, batch=batch,
+               seed=3)
+
+
+@pytest.mark.parametrize("dist", ["normal", "loguniform"])
+def test_lowp_wide_range_inputs(handle, torch_cuda, dist):
+    import portblas_amd as pb
+
+    for prec in ("bf16x3", "bf16x4", "fp8_e4m3"):
+        check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], "n", "n", 4096, 4096, 512, dist=dist, seed=5)
+
+
+def test_bf16x4_not_less_accurate_than_bf16x3(handle, torch_cuda):
+    """The extra lo*lo term can only shrink the representation error: on
+    normal inputs the mean componentwise error must drop."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    m = n = 4096
+    k = 2048
+    A = make(torch, m * k, 11, "normal")
+    B = make(torch, k * n, 12, "normal")
+    ref = cm(A, m, k, m).double() @ cm(B, k, n, k).double()
+    absprod = cm(A, m, k, m).double().abs() @ cm(B, k, n, k).double().abs()
+    errs = {}
+    for prec in ("bf16x3", "bf16x4"):
+        C = torch.zeros(m * n, device="cuda")
+        pb._gemm_ex(handle, pb.gemm_precision_t[prec], "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+        errs[prec] = ((cm(C, m, n, m).double() - ref).abs() / absprod).mean().item()
+    assert errs["bf16x4"] < 0.9 * errs["bf16x3"], errs
+
+
+@pytest.mark.parametrize("m,n,k", [(4096, 4096, 512), (300, 200, 100)])
+def test_pack_once_matches_gemm_ex(handle, torch_cuda, m, n, k):
+    """pb_pack_bf16x3 + pb_gemm_bf16x3_packed computes exactly what
+    pb_gemm_ex(BF16X3) computes (same split, same kernels)."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    A = make(torch, m * k, 21)
+    B = make(torch, k * n, 22)
+    C1 = torch.zeros(m * n, device="cuda")
+    C2 = torch.zeros(m * n, device="cuda")
+    pb._gemm_ex(handle, pb.gemm_precision_t.bf16x3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C1, m)
+    big = m * n >= 4096 * 4096
+    ldp = ((k + 63) // 64) * 64 if big else ((k + 7) // 8) * 8
+    ah = torch.empty(m * ldp, dtype=torch.int16, device="cuda")
+    al = torch.empty_like(ah)
+    bh = torch.empty(n * ldp, dtype=torch.int16, device="cuda")
+    bl = torch.empty_like(bh)
+    pb.check(L.pb_pack_bf16x3(handle.handle, 0, b"n", m, k, A.data_ptr(), m, ah.data_ptr(), al.data_ptr(), ldp))
+    pb.check(L.pb_pack_bf16x3(handle.handle, 1, b"n", n, k, B.data_ptr(), k, bh.data_ptr(), bl.data_ptr(), ldp))
+    pb.check(L.pb_gemm_bf16x3_packed(handle.handle, m, n, k, 1.0, ah.data_ptr(), al.data_ptr(), ldp,
+                                     bh.data_ptr(), bl.data_ptr(), ldp, 0.0, C2.data_ptr(), m))
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    # the split itself: hi + lo reproduces x to 2^-16 relative
+    hi = (ah.view(m, ldp)[:, :k].int() << 16).view(torch.float32)
+    lo = (al.view(m, ldp)[:, :k].int() << 16).view(torch.float32)
+    x = cm(A, m, k, m)
+    assert ((hi.double() + lo.double() - x.double()).abs() <= 2.0 ** -16 * x.double().abs() + 1e-38).all()
+    assert (ah.view(m, ldp)[:, k:] == 0).all() and (al.view(m, ldp)[:, k:] == 0).all()
+
+
+@pytest.mark.parametrize("ta,tb", [("n", "n"), ("t", "t")])
+@pytest.mark.parametrize("m,n,k", [(96, 80, 70), (4096, 4096, 128)])
+def test_gemm_bf16_inputs(handle, torch_cuda, ta, tb, m, n, k):
+    """BF16 inputs are exact in FP32: only accumulation error remains."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    ar, ac = (m, k) if ta == "n" else (k, m)
+    br, bc = (k, n) if tb == "n" else (n, k)
+    A = make(torch, ar * ac, 31).to(torch.bfloat16)
+    B = make(torch, br * bc, 32).to(torch.bfloat16)
+    C = make(torch, m * n, 33)
+    C0 = C.clone()
+    pb._gemm_bf16(handle, ta, tb, m, n, k, 2.0, A, ar, B, br, -1.0, C, m)
+    oa = cm(A, ar, ac, ar).double()
+    ob = cm(B, br, bc, br).double()
+    oa = oa if ta == "n" else oa.t()
+    ob = ob if tb == "n" else ob.t()
+    ref = 2.0 * (oa @ ob) - cm(C0, m, n, m).double()
+    tol = 1.01 * (k + 2) * U * 2.0 * (oa.abs() @ ob.abs()) + 2 * U * cm(C0, m, n, m).double().abs()
+    err = (cm(C, m, n, m).double() - ref).abs()
+    assert (err <= tol + 1e-30).all(), (err / tol).max().item()

@@ -83,7 +83,8 @@ def test_gpu_mlp_forward_backward_adam(handle, torch_cuda):
     P = lambda ts: (ctypes.c_void_p * len(ts))(*[q.data_ptr() for q in ts])
     n = len(SIZES) - 1
     pb.check(L.pb_mlp_forward(handle.handle, n, sizes, P(W), P(Bv), P(acts), B))
-    pb.check(L.pb_mlp_backward(handle.handle, n, sizes, P(W), P(acts), f32(t).data_ptr(), f32(m).data_ptr(),
+    td, md = f32(t), f32(m)  # held alive: a temporary's block would be reused by the next allocation
+    pb.check(L.pb_mlp_backward(handle.handle, n, sizes, P(W), P(acts), td.data_ptr(), md.data_ptr(),
                                P(dW), P(dB), loss.data_ptr(), scratch.data_ptr(), B))
     handle.wait()
     # oracle on the FP32-rounded parameters
