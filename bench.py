@@ -104,6 +104,7 @@ def precision_study(torch, pb, h, timer, N, kind):
     for name, prec in (("fp32_mfma", pb.gemm_precision_t.fp32_mfma),
                        ("fp32_simt", pb.gemm_precision_t.fp32_simt),
                        ("bf16x3", pb.gemm_precision_t.bf16x3),
+                       ("bf16x4", pb.gemm_precision_t.bf16x4),
                        ("bf16", pb.gemm_precision_t.bf16),
                        ("fp8_e4m3", pb.gemm_precision_t.fp8_e4m3)):
         pb._gemm_ex(h, prec, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N)
@@ -179,7 +180,7 @@ def main():
     achieved = mfma_flop / (t_kern * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["bf16"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK["bf16"], "traffic": None,
-                "kernel": "gemm_kernel<bf16_t, 256x256x64 or selected tile> (3 BF16 terms)",
+                "kernel": "mt_kernel<32,3>: 256x256 tile, BF16x3 (3 BF16 MFMA terms fused per K step)",
                 "kernel_ms": t_kern, "flop_per_launch": mfma_flop}
     log(f"bf16 gemm kernel alone: {t_kern:.3f} ms, {achieved:.0f} TFLOP/s of BF16 MFMA "
         f"({100 * achieved / PEAK['bf16']:.1f}% of peak)")
@@ -193,7 +194,8 @@ def main():
                 log(f"precision {kind:10s} {name:9s} max_rel {e['max_rel']:.3e} mean_rel {e['mean_rel']:.3e} "
                     f"normwise {e['normwise']:.3e}")
         # other variants' throughput on the headline inputs
-        for name, prec in (("bf16", pb.gemm_precision_t.bf16), ("fp8_e4m3", pb.gemm_precision_t.fp8_e4m3),
+        for name, prec in (("bf16x4", pb.gemm_precision_t.bf16x4), ("bf16", pb.gemm_precision_t.bf16),
+                           ("fp8_e4m3", pb.gemm_precision_t.fp8_e4m3),
                            ("fp32_simt", pb.gemm_precision_t.fp32_simt)):
             t = timer(lambda: pb._gemm_ex(h, prec, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N), reps=3)
             res[f"sgemm_{name}_tflops"] = flop / t / 1e9

@@ -160,14 +160,17 @@ pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t precision, char transa
                        int64_t ldc, int64_t stridec, int64_t batch_size);
 
 /* Split once, multiply many times (no reference counterpart): writes op(X)
- * K-contiguous as BF16 hi = bf16(x) and lo = bf16(x - hi), row pitch ldp
- * (>= k, multiple of 8).  operand 0: X is A and op(A) is rows x k;
- * operand 1: X is B and op(B) is k x rows. */
+ * as BF16 hi = bf16(x) and lo = bf16(x - hi) in the packed format of the
+ * BF16x3 kernel: K-slab-major, element (r, k) of the rows x ldp operand at
+ * p[(k / 16) * rows * 16 + r * 16 + k % 16], zeros for k in [k, ldp); ldp a
+ * multiple of 64.  operand 0: X is A and op(A) is rows x k; operand 1: X is
+ * B and op(B) is k x rows. */
 pb_status_t pb_pack_bf16x3(pb_handle_t h, int operand, char trans, int64_t rows, int64_t k,
                            const float* x, int64_t ldx, uint16_t* hi, uint16_t* lo,
                            int64_t ldp);
 /* C = alpha * (A_hi B_lo + A_lo B_hi + A_hi B_hi) + beta * C on packed
- * operands from pb_pack_bf16x3 (the arithmetic of PB_GEMM_BF16X3). */
+ * operands from pb_pack_bf16x3 (the arithmetic of PB_GEMM_BF16X3); both
+ * operands share one packed pitch (ldpa == ldpb). */
 pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k, float alpha,
                                   const uint16_t* a_hi, const uint16_t* a_lo, int64_t ldpa,
                                   const uint16_t* b_hi, const uint16_t* b_lo, int64_t ldpb,

@@ -1,6 +1,6 @@
 """Autotuning + learned-selector pipeline (GPU).
 
-    python -m portblas_amd.autotune --train 20000 --heldout 2000 \
+    python -m portblas_amd.autotune --train 20000 --heldout 2000 \\
         --out portblas_amd/selector_mi355x.npz --report gpurun_out/tune_report.json
 
 1. draws training and held-out shapes (M, N, K log-uniform in [16, 16384],
@@ -12,6 +12,10 @@
    library's own kernels (FP32), then evaluates it in BF16 on the held-out
    shapes against the exhaustive optimum and against the built-in rule;
 4. saves the network and a JSON report.
+
+Timing datasets can be collected in pieces (``--collect-only`` with
+``--train-files`` / ``--heldout-file`` naming where to save them) and reused:
+a file that exists is loaded instead of re-timed.
 The precedent is tools/auto_tuner (gemm_tuner.hpp:99-180: time every config,
 keep the best); the network replaces the hand-written buckets of
 src/interface/blas3/backend/amd_gpu.hpp:52-288.
@@ -27,17 +31,45 @@ from pathlib import Path
 import numpy as np
 
 
+def _enc(s):
+    return [ord(s[0]), ord(s[1]), *s[2:]]
+
+
+def _dec(row):
+    return (chr(int(row[0])), chr(int(row[1])), *(int(v) for v in row[2:]))
+
+
+def _timed_set(tuning, tuner_box, n, seed, max_flop, reps, path, progress_every):
+    """(X, T, shapes) for n shapes drawn with `seed`, loaded from `path` when
+    it exists, else timed (and saved to `path` when given)."""
+    if path and Path(path).exists():
+        z = np.load(path, allow_pickle=False)
+        print(f"[tune] loaded {path}: {len(z['X'])} shapes", flush=True)
+        return z["X"], z["T"], [_dec(r) for r in z["shapes"]]
+    shapes = tuning.sample_shapes(n, seed=seed, max_flop=max_flop)
+    t0 = time.time()
+    X, T = tuning.collect(tuner_box(), shapes, progress=progress_every, reps=reps)
+    print(f"[tune] timed {n} shapes x {T.shape[1]} configs in {time.time() - t0:.0f} s", flush=True)
+    if path:
+        Path(path).parent.mkdir(parents=True, exist_ok=True)
+        np.savez(path, X=X, T=T, shapes=np.array([_enc(s) for s in shapes], dtype=np.int64))
+    return X, T, shapes
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
-    ap.add_argument("--train", type=int, default=20000)
+    ap.add_argument("--train", type=int, default=20000, help="training shapes per train file")
     ap.add_argument("--heldout", type=int, default=2000)
-    ap.add_argument("--steps", type=int, default=4000)
+    ap.add_argument("--steps", type=int, default=6000)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--max-flop", type=float, default=2.0 ** 35)
-    ap.add_argument("--reps", type=int, default=2)
+    ap.add_argument("--reps", type=int, default=3, help="timed repetitions per config")
+    ap.add_argument("--train-files", nargs="*", default=[],
+                    help="timing datasets of training shapes (file i uses seed + i)")
+    ap.add_argument("--heldout-file", default=None)
+    ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
-    ap.add_argument("--dataset", default=None, help="save/load the timing dataset (npz)")
     args = ap.parse_args(argv)
 
     import torch
@@ -47,27 +79,29 @@ def main(argv=None):
 
     h = pb.SB_Handle()
     nconf = tuning.num_configs()
+    tuner = []
+
+    def tuner_box():
+        if not tuner:
+            tuner.append(tuning.Autotuner(h))
+        return tuner[0]
+
     t0 = time.time()
-    if args.dataset and Path(args.dataset).exists():
-        z = np.load(args.dataset, allow_pickle=False)
-        Xtr, Ttr, Xte, Tte = z["Xtr"], z["Ttr"], z["Xte"], z["Tte"]
-        te_shapes = [tuple(s) for s in z["te_shapes"].tolist()]
-        print(f"[tune] loaded dataset {args.dataset}", flush=True)
-    else:
-        tr_shapes = tuning.sample_shapes(args.train, seed=args.seed, max_flop=args.max_flop)
-        te_shapes = tuning.sample_shapes(args.heldout, seed=args.seed + 1000, max_flop=args.max_flop)
-        tuner = tuning.Autotuner(h)
-        prog = max(1, args.train // 20)
-        Xtr, Ttr = tuning.collect(tuner, tr_shapes, progress=prog)
-        Xte, Tte = tuning.collect(tuner, te_shapes, progress=max(1, args.heldout // 10))
-        del tuner
-        torch.cuda.empty_cache()
-        if args.dataset:
-            enc = lambda s: [ord(s[0]), ord(s[1]), *s[2:]]
-            np.savez(args.dataset, Xtr=Xtr, Ttr=Ttr, Xte=Xte, Tte=Tte,
-                     te_shapes=np.array([enc(s) for s in te_shapes], dtype=np.int64))
+    train_files = args.train_files or [None]
+    parts = [_timed_set(tuning, tuner_box, args.train, args.seed + i, args.max_flop, args.reps, f,
+                        max(1, args.train // 20)) for i, f in enumerate(train_files)]
+    Xtr = np.concatenate([p[0] for p in parts])
+    Ttr = np.concatenate([p[1] for p in parts])
+    if args.collect_only and not args.heldout_file:
+        return {"train_shapes": int(len(Xtr))}
+    Xte, Tte, te_shapes = _timed_set(tuning, tuner_box, args.heldout, args.seed + 1000, args.max_flop,
+                                     args.reps, args.heldout_file, max(1, args.heldout // 10))
+    if args.collect_only:
+        return {"train_shapes": int(len(Xtr)), "heldout_shapes": int(len(Xte))}
+    tuner.clear()
+    torch.cuda.empty_cache()
     t_collect = time.time() - t0
-    print(f"[tune] timing: {len(Xtr)} + {len(Xte)} shapes x {nconf} configs in {t_collect:.0f} s", flush=True)
+    print(f"[tune] datasets: {len(Xtr)} + {len(Xte)} shapes x {nconf} configs ({t_collect:.0f} s)", flush=True)
 
     sel = tuning.Selector.init(sizes=(tuning.NUM_FEATURES, 64, 64, 64, nconf), seed=args.seed)
     t1 = time.time()
@@ -79,22 +113,21 @@ def main(argv=None):
     chosen_mlp = np.argmin(pred, axis=1)
     # built-in rule (no learned selector) for the same shapes
     tuning.uninstall(h)
-    def enc_shape(s):
-        ta, tb = (chr(s[0]), chr(s[1])) if isinstance(s[0], (int, np.integer)) else (s[0], s[1])
-        return ta, tb, int(s[2]), int(s[3]), int(s[4]), int(s[5])
-    shapes = [enc_shape(s) for s in te_shapes]
-    chosen_rule = np.array([tuning.select(h, *s) for s in shapes])
+    chosen_rule = np.array([tuning.select(h, *s) for s in te_shapes])
     # the handle's own dispatch with the network installed agrees with the GPU inference
     sel.install(h)
-    chosen_host = np.array([tuning.select(h, *s) for s in shapes])
+    chosen_host = np.array([tuning.select(h, *s) for s in te_shapes])
+    tuning.uninstall(h)
     agree = float(np.mean(chosen_host == chosen_mlp))
+    best_single = int(np.argmin(np.log(Ttr).mean(axis=0)))
     rep = {
         "train_shapes": int(len(Xtr)), "heldout_shapes": int(len(Xte)), "configs": nconf,
         "timing_seconds": t_collect, "train_seconds": t_train, "final_loss": hist[-1][1] if hist else None,
         "mlp_bf16": tuning.evaluate(chosen_mlp, Tte),
         "builtin_rule": tuning.evaluate(chosen_rule, Tte),
+        "best_single_config": tuning.evaluate(np.full(len(Tte), best_single), Tte),
+        "best_single_config_name": tuning.config_name(best_single),
         "host_dispatch_agrees_with_gpu_inference": agree,
-        "best_single_config": tuning.evaluate(np.full(len(Tte), int(np.argmin(np.log(Tte).mean(axis=0)))), Tte),
     }
     print(json.dumps(rep, indent=1), flush=True)
     if args.out:

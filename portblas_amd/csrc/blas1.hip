@@ -206,53 +206,76 @@ __device__ __forceinline__ void block_best(T& v, int64_t& i) {
   }
 }
 
-template <bool IS_MAX, typename T, typename IndexT, bool VEC>
+// Per-workgroup best (value, index).  A thread visits its elements in
+// increasing index order, so inside one thread "strictly better" already
+// implements the lowest-index tie rule, and a running value below every
+// candidate (-1 for max; +inf for min, which like the reference's FLT_MAX
+// initial value lets FLT_MAX win and never inf) stands for the reference's
+// initial tuple.  IDX is the in-thread index type: 32-bit whenever n fits.
+// Threads and workgroups are then combined with the full better() rule.
+template <bool IS_MAX, typename T>
+__device__ __forceinline__ bool thread_better(T v, T bv) {
+  return IS_MAX ? (v > bv) : (v < bv);
+}
+
+template <bool IS_MAX, typename T, typename IDX, bool VEC>
 __global__ __launch_bounds__(kBlock) void iamax_kernel(
     int64_t n, const T* __restrict__ x, int64_t incx, T* __restrict__ pv,
     int64_t* __restrict__ pi) {
-  T bv = init_val<IS_MAX, T>();
-  int64_t bi = kSentinel;
+  T bv = IS_MAX ? T(-1) : std::numeric_limits<T>::infinity();
+  IDX bi = IDX(-1);
   if constexpr (VEC) {
     using V = typename Vec16<T>::type;
     constexpr int VN = Vec16<T>::n;
     const int64_t nvec = n / VN;
     const V* xv = reinterpret_cast<const V*>(x);
     const int64_t step = int64_t(gridDim.x) * kBlock * kUnroll;
-    for (int64_t base = int64_t(blockIdx.x) * kBlock * kUnroll + threadIdx.x;
-         base < nvec; base += step) {
+    int64_t base = int64_t(blockIdx.x) * kBlock * kUnroll + threadIdx.x;
+    // Full iterations: no bounds checks inside.
+    for (; base + (kUnroll - 1) * kBlock < nvec; base += step) {
       V a[kUnroll];
 #pragma unroll
-      for (int u = 0; u < kUnroll; ++u) {
-        const int64_t iv = base + u * kBlock;
-        if (iv < nvec) a[u] = pbdev::ld_stream(xv + iv);
-      }
+      for (int u = 0; u < kUnroll; ++u) a[u] = pbdev::ld_stream(xv + base + u * kBlock);
+#pragma unroll
+      for (int u = 0; u < kUnroll; ++u)
+#pragma unroll
+        for (int j = 0; j < VN; ++j) {
+          const T v = fabs(a[u][j]);
+          if (thread_better<IS_MAX>(v, bv)) { bv = v; bi = IDX((base + u * kBlock) * VN + j); }
+        }
+    }
+    if (base < nvec) {
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {
         const int64_t iv = base + u * kBlock;
         if (iv < nvec) {
+          const V a = pbdev::ld_stream(xv + iv);
 #pragma unroll
           for (int j = 0; j < VN; ++j) {
-            const T v = fabs(a[u][j]);
-            const int64_t idx = iv * VN + j;
-            if (better<IS_MAX>(v, idx, bv, bi)) { bv = v; bi = idx; }
+            const T v = fabs(a[j]);
+            if (thread_better<IS_MAX>(v, bv)) { bv = v; bi = IDX(iv * VN + j); }
           }
         }
       }
     }
+    // Scalar tail: indices above every vector index, visited last.
     if (blockIdx.x == 0 && threadIdx.x < n - nvec * VN) {
       const int64_t idx = nvec * VN + threadIdx.x;
       const T v = fabs(x[idx]);
-      if (better<IS_MAX>(v, idx, bv, bi)) { bv = v; bi = idx; }
+      if (thread_better<IS_MAX>(v, bv)) { bv = v; bi = IDX(idx); }
     }
   } else {
     const int64_t step = int64_t(gridDim.x) * kBlock;
     for (int64_t i = int64_t(blockIdx.x) * kBlock + threadIdx.x; i < n; i += step) {
       const T v = fabs(pbdev::ld_stream(x + i * incx));
-      if (better<IS_MAX>(v, i, bv, bi)) { bv = v; bi = i; }
+      if (thread_better<IS_MAX>(v, bv)) { bv = v; bi = IDX(i); }
     }
   }
-  block_best<IS_MAX>(bv, bi);
-  if (threadIdx.x == 0) { pv[blockIdx.x] = bv; pi[blockIdx.x] = bi; }
+  T v = init_val<IS_MAX, T>();
+  int64_t i = kSentinel;
+  if (bi != IDX(-1)) { v = bv; i = int64_t(bi); }
+  block_best<IS_MAX>(v, i);
+  if (threadIdx.x == 0) { pv[blockIdx.x] = v; pi[blockIdx.x] = i; }
 }
 
 template <bool IS_MAX, typename T, typename IndexT>
@@ -359,12 +382,15 @@ pb_status_t iamax_launch(pb_handle_t h, int64_t n, const T* x, int64_t incx,
   PB_TRY(pb::workspace(h, (sizeof(T) + sizeof(int64_t)) * grid + 256, &ws));
   int64_t* pi = static_cast<int64_t*>(ws);
   T* pv = reinterpret_cast<T*>(pi + grid);
-  if (vec)
-    hipLaunchKernelGGL((iamax_kernel<IS_MAX, T, IndexT, true>), dim3(grid),
-                       dim3(kBlock), 0, h->stream, n, x, incx, pv, pi);
-  else
-    hipLaunchKernelGGL((iamax_kernel<IS_MAX, T, IndexT, false>), dim3(grid),
-                       dim3(kBlock), 0, h->stream, n, x, incx, pv, pi);
+  const bool idx32 = n <= int64_t(std::numeric_limits<int32_t>::max());
+#define PB_IAMAX(IDX, V)                                                                   \
+  hipLaunchKernelGGL((iamax_kernel<IS_MAX, T, IDX, V>), dim3(grid), dim3(kBlock), 0, h->stream, \
+                     n, x, incx, pv, pi)
+  if (vec && idx32) PB_IAMAX(int32_t, true);
+  else if (vec) PB_IAMAX(int64_t, true);
+  else if (idx32) PB_IAMAX(int32_t, false);
+  else PB_IAMAX(int64_t, false);
+#undef PB_IAMAX
   hipLaunchKernelGGL((iamax_fold_kernel<IS_MAX, T, IndexT>), dim3(1), dim3(kBlock), 0, h->stream,
                      pv, pi, grid, rs);
   PB_CHECK_LAUNCH();

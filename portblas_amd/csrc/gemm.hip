@@ -6,6 +6,7 @@
 // (SIMT) kernel used as the FP32-SIMT variant of pb_gemm_ex.
 #include "gemm_launch.hpp"
 #include "gemm_front.hpp"
+#include "gemm_small.hpp"
 
 namespace pbgemm {
 
@@ -204,6 +205,8 @@ pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt) {
   p.c_vec = !g.interleaved && aligned16(g.c) && g.ldc % 4 == 0 &&
             (g.batch == 1 || g.sc % 4 == 0);
   const bool kca = g.ta == 't', kcb = g.tb == 'n';
+  if (!simt && small::eligible(h, g.M, g.N, g.batch, g.interleaved))
+    return small::launch<T>(h, p, vec);
   if constexpr (sizeof(T) == 4) {
     if (simt) return launch_simt(h, p, kca, kcb, vec);
     const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 32);

@@ -80,14 +80,24 @@ __global__ void fill_zero_u32(unsigned int* p, int n) {
 
 enum PackMode { kPackBf16 = 0, kPackBf16Split = 1, kPackFp8 = 2 };
 
+// Index of packed element (r, k) of batch b.  slab == 0: row-major
+// [batch][rows][kp]; slab > 0: K-slab-major [batch][kp / slab][rows][slab],
+// the layout the 256 x 256 kernel streams (one K step of 32 consecutive rows
+// is one contiguous kilobyte).
+__device__ __forceinline__ int64_t packed_index(int64_t b, int64_t r, int64_t k, int64_t rows,
+                                                int64_t kp, int slab) {
+  if (slab == 0) return (b * rows + r) * kp + k;
+  return ((b * (kp / slab) + k / slab) * rows + r) * slab + (k % slab);
+}
+
 // Writes op(X) (rows x K, element (r, k) at x[r*rs + k*ks], batch b at
-// + b*stride) as dst[b][r][k] with row pitch kp (zero for k in [K, kp)).
+// + b*stride) in the packed layout (packed_index; zero for k in [K, kp)).
 // 64 x 64 tiles through LDS: reads walk X's contiguous dimension, writes walk K.
 template <int MODE>
 __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, int64_t rs,
                                                    int64_t ks, int64_t rows, int64_t K,
-                                                   int64_t stride, int64_t kp, void* dst_hi,
-                                                   void* dst_lo, const float* scales,
+                                                   int64_t stride, int64_t kp, int slab,
+                                                   void* dst_hi, void* dst_lo, const float* scales,
                                                    int scale_idx) {
   __shared__ float tile[64][65];
   const int64_t r0 = int64_t(blockIdx.x) * 64;
@@ -107,7 +117,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
     const int rr = e / 64, kk = e % 64;
     const int64_t r = r0 + rr, k = k0 + kk;
     if (r >= rows || k >= kp) continue;
-    const int64_t o = (b * rows + r) * kp + k;
+    const int64_t o = packed_index(b, r, k, rows, kp, slab);
     const float v = tile[rr][kk];
     if constexpr (MODE == kPackFp8) {
       static_cast<uint8_t*>(dst_hi)[o] = pblowp::f32_to_e4m3(v * s);
@@ -139,9 +149,9 @@ __device__ __forceinline__ void pack_convert(float v, uint16_t& hi, uint16_t& lo
 template <int MODE, bool KC>
 __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__ x, int64_t rs,
                                                        int64_t ks, int64_t rows, int64_t K,
-                                                       int64_t stride, int64_t kp, void* dst_hi,
-                                                       void* dst_lo, const float* scales,
-                                                       int scale_idx) {
+                                                       int64_t stride, int64_t kp, int slab,
+                                                       void* dst_hi, void* dst_lo,
+                                                       const float* scales, int scale_idx) {
   const int64_t r0 = int64_t(blockIdx.x) * 64;
   const int64_t k0 = int64_t(blockIdx.y) * 64;
   const int64_t b = blockIdx.z;
@@ -165,7 +175,7 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
 #pragma unroll
         for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? xb[r * rs + k + e] : 0.f;
       }
-      const int64_t o = (b * rows + r) * kp + k;
+      const int64_t o = packed_index(b, r, k, rows, kp, slab);
       uint16_t h[4], l[4];
       uint8_t q[4];
 #pragma unroll
@@ -212,7 +222,7 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
       const int kg = item % 8, rr = item / 8;
       const int64_t r = r0 + rr, k = k0 + 8 * kg;
       if (r >= rows || k >= kp) continue;
-      const int64_t o = (b * rows + r) * kp + k;
+      const int64_t o = packed_index(b, r, k, rows, kp, slab);
       uint16_t h[8], l[8];
       uint8_t q[8];
 #pragma unroll
@@ -350,7 +360,7 @@ bool mt_eligible(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch) 
   if (batch > 65535 || K <= 0) return false;
   const int64_t tiles = pb::ceil_div(M, mt::kBM) * pb::ceil_div(N, mt::kBN) * batch;
   const double waste = double(tiles) * mt::kBM * mt::kBN / (double(M) * double(N) * double(batch));
-  return tiles >= h->num_cus && waste < 1.15 && mt::fits(M, N, pb::align_up(size_t(K), 64));
+  return tiles >= h->num_cus && waste < 1.15 && mt::fits(M, N, pb::align_up(size_t(K), 128));
 }
 
 pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall<float>& g) {
@@ -358,7 +368,10 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   const bool fp8 = prec == PB_GEMM_FP8_E4M3;
   const bool split = prec == PB_GEMM_BF16X3 || prec == PB_GEMM_BF16X4;
   const bool use_mt = !fp8 && mt_eligible(h, M, N, K, batch);
-  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1), use_mt ? 64 : (fp8 ? 16 : 8));
+  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1),
+                                  use_mt ? (split ? mt::kKAlign<16> : mt::kKAlign<32>)
+                                         : (fp8 ? 16 : 8));
+  const int slab = use_mt ? (split ? 16 : 32) : 0;  // the 256 x 256 kernel's K step
   const size_t esz = fp8 ? 1 : 2;
   const int nparts = split ? 2 : 1;
   const size_t abytes = pb::align_up(size_t(M) * kp * esz * batch, 256);
@@ -419,10 +432,10 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   do {                                                                                      \
     if (kc)                                                                                 \
       hipLaunchKernelGGL((pack_vec_kernel<MODE, true>), grid, dim3(256), 0, h->stream, x, rs, \
-                         ks, rows, K, stride, kp, hi, lo, scales, sidx);                    \
+                         ks, rows, K, stride, kp, slab, hi, lo, scales, sidx);              \
     else                                                                                    \
       hipLaunchKernelGGL((pack_vec_kernel<MODE, false>), grid, dim3(256), 0, h->stream, x,  \
-                         rs, ks, rows, K, stride, kp, hi, lo, scales, sidx);                \
+                         rs, ks, rows, K, stride, kp, slab, hi, lo, scales, sidx);          \
   } while (0)
       if (split) PB_PACKV(kPackBf16Split);
       else if (prec == PB_GEMM_BF16) PB_PACKV(kPackBf16);
@@ -432,13 +445,13 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
     }
     if (split)
       hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
-                         rows, K, stride, kp, hi, lo, scales, sidx);
+                         rows, K, stride, kp, slab, hi, lo, scales, sidx);
     else if (prec == PB_GEMM_BF16)
       hipLaunchKernelGGL(pack_kernel<kPackBf16>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
-                         stride, kp, hi, lo, scales, sidx);
+                         stride, kp, slab, hi, lo, scales, sidx);
     else
       hipLaunchKernelGGL(pack_kernel<kPackFp8>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
-                         stride, kp, hi, lo, scales, sidx);
+                         stride, kp, slab, hi, lo, scales, sidx);
   };
   pack(g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0);
   pack(g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1);
@@ -469,9 +482,9 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
     q.a[0] = ah; q.a[1] = al; q.b[0] = bh; q.b[1] = bl;
     q.a_stride = M * kp; q.b_stride = N * kp;
     q.c = g.c; q.ldc = g.ldc; q.c_stride = g.sc; q.batch = int(batch); q.c_vec = c_vec;
-    if (prec == PB_GEMM_BF16X3) return mt::launch<32, 3>(h, q);
-    if (prec == PB_GEMM_BF16X4) return mt::launch<32, 4>(h, q);
-    return mt::launch<64, 1>(h, q);
+    if (prec == PB_GEMM_BF16X3) return mt::launch<16, 3>(h, q);
+    if (prec == PB_GEMM_BF16X4) return mt::launch<16, 4>(h, q);
+    return mt::launch<32, 1>(h, q);
   }
   GemmProblem<bf16_t> p{};
   p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;
@@ -568,16 +581,28 @@ pb_status_t pb_pack_bf16x3(pb_handle_t h, int operand, char trans, int64_t rows,
   PB_REQUIRE(rows >= 0 && k >= 0, "negative dimension");
   if (rows == 0 || k == 0) return PB_STATUS_SUCCESS;
   PB_REQUIRE(x && hi && lo, "null pointer");
-  PB_REQUIRE(ldp >= k && ldp % 8 == 0, "ldp must be >= k and a multiple of 8");
+  PB_REQUIRE(ldp >= k && ldp % mt::kKAlign<16> == 0, "ldp must be >= k and a multiple of %d",
+             mt::kKAlign<16>);
   // A: op(A) is rows x k; B: op(B) is k x rows, addressed as (row, k).
   int64_t rs, ks;
   if (operand == 0) { rs = t == 'n' ? 1 : ldx; ks = t == 'n' ? ldx : 1; }
   else { rs = t == 'n' ? ldx : 1; ks = t == 'n' ? 1 : ldx; }
   PB_REQUIRE(ldx >= ((rs == 1) ? rows : k), "leading dimension too small");
   dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(ldp, 64)), 1u};
-  hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks, rows,
-                     k, int64_t(0), ldp, static_cast<void*>(hi), static_cast<void*>(lo),
-                     static_cast<const float*>(nullptr), 0);
+  const bool vec = aligned16(x) && (rs == 1 ? ks : rs) % 4 == 0 && (rs == 1 || ks == 1);
+  void* dh = static_cast<void*>(hi);
+  void* dl = static_cast<void*>(lo);
+  const float* none = nullptr;
+  constexpr int slab = 16;
+  if (vec && ks == 1)
+    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, true>), grid, dim3(256), 0, h->stream, x,
+                       rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0);
+  else if (vec)
+    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, false>), grid, dim3(256), 0, h->stream,
+                       x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0);
+  else
+    hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
+                       rows, k, int64_t(0), ldp, slab, dh, dl, none, 0);
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }
@@ -592,32 +617,21 @@ pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k
   if (m == 0 || n == 0) return PB_STATUS_SUCCESS;
   PB_REQUIRE(c != nullptr && ldc >= m, "bad C");
   PB_REQUIRE(k == 0 || (a_hi && a_lo && b_hi && b_lo), "null operand");
-  PB_REQUIRE(ldpa >= k && ldpb >= k, "packed pitch smaller than k");
-  if (alpha == 0.f) return sgemm_alpha_zero(h, m, n, beta, c, ldc, 0, 1);
-  const bool aligned = aligned16(a_hi) && aligned16(a_lo) && aligned16(b_hi) && aligned16(b_lo);
-  if (ldpa == ldpb && ldpa % 32 == 0 && aligned && mt_eligible(h, m, n, k, 1) &&
-      mt::fits(m, n, ldpa)) {
-    // Columns [k, ldp) of the packed rows are zero (pb_pack_bf16x3 writes them).
-    mt::Problem q{};
-    q.M = int(m); q.N = int(n); q.kp = int(ldpa);
-    q.alpha = alpha; q.beta = beta; q.alpha_dev = nullptr;
-    q.a[0] = a_hi; q.a[1] = a_lo; q.b[0] = b_hi; q.b[1] = b_lo;
-    q.c = c; q.ldc = ldc; q.batch = 1; q.c_vec = aligned16(c) && ldc % 4 == 0;
-    return mt::launch<32, 3>(h, q);
-  }
-  GemmProblem<bf16_t> p{};
-  p.M = m; p.N = n; p.K = k; p.alpha = alpha; p.beta = beta;
-  p.nterms = 3;
-  p.a[0] = a_hi; p.b[0] = b_lo;
-  p.a[1] = a_lo; p.b[1] = b_hi;
-  p.a[2] = a_hi; p.b[2] = b_hi;
-  p.a_rs = ldpa; p.a_ks = 1; p.a_stride = 0;
-  p.b_rs = ldpb; p.b_ks = 1; p.b_stride = 0;
-  p.c = c; p.c_rs = 1; p.ldc = ldc; p.c_stride = 0; p.batch = 1;
-  p.c_vec = aligned16(c) && ldc % 4 == 0;
-  const bool vec = ldpa % 8 == 0 && ldpb % 8 == 0 && aligned16(a_hi) && aligned16(a_lo) &&
-                   aligned16(b_hi) && aligned16(b_lo);
-  return run_lowp<bf16_t>(h, -1, p, true, true, vec);
+  PB_REQUIRE(ldpa == ldpb && ldpa >= k && ldpa % mt::kKAlign<16> == 0,
+             "packed pitches must match, be >= k and a multiple of %d", mt::kKAlign<16>);
+  // alpha == 0 or an empty contraction: C <- beta C.
+  if (alpha == 0.f || k == 0) return sgemm_alpha_zero(h, m, n, beta, c, ldc, 0, 1);
+  PB_REQUIRE(aligned16(a_hi) && aligned16(a_lo) && aligned16(b_hi) && aligned16(b_lo),
+             "packed operands must be 16-byte aligned");
+  PB_REQUIRE(mt::fits(m, n, ldpa), "packed operands too large for 32-bit offsets");
+  // The packed format is the fused kernel's (K-slab-major, zero K padding,
+  // written by pb_pack_bf16x3); it runs for every shape.
+  mt::Problem q{};
+  q.M = int(m); q.N = int(n); q.kp = int(ldpa);
+  q.alpha = alpha; q.beta = beta; q.alpha_dev = nullptr;
+  q.a[0] = a_hi; q.a[1] = a_lo; q.b[0] = b_hi; q.b[1] = b_lo;
+  q.c = c; q.ldc = ldc; q.batch = 1; q.c_vec = aligned16(c) && ldc % 4 == 0;
+  return mt::launch<16, 3>(h, q);
 }
 
 pb_status_t pb_gemm_bf16(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,

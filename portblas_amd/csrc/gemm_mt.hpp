@@ -1,8 +1,10 @@
 // Large-tile BF16 MFMA kernel for packed operands (the hot GEMM of the
 // BF16x3 / BF16 paths at large M, N).
 //
-// Operands are the pre-pass output: op(A) as [M][kp] and op(B) as [N][kp]
-// BF16, K-contiguous, kp a multiple of 64 with zeros in [K, kp).  Up to two
+// Operands are the pre-pass output: op(A) (M x kp) and op(B) (N x kp) in
+// BF16, K-slab-major ([kp / BK][rows][BK], BK the kernel's K step), kp a
+// multiple of 4 K steps (64 for the split formats, 128 for BF16) with zeros
+// in [K, kp).  Up to two
 // parts per operand (hi, lo); TERMS products are accumulated per K step:
 //   TERMS 1: Ah Bh
 //   TERMS 3: Ah Bl + Al Bh + Ah Bh           (BF16x3)
@@ -10,21 +12,26 @@
 //
 // Workgroup: 256 threads = 4 waves (2 x 2), C tile 256 x 256, wave tile
 // 128 x 128 = 4 x 4 v_mfma_f32_32x32x16_bf16 tiles (256 FP32 accumulators per
-// lane).  Per K step of BK:
-//   * global -> VGPR: every thread loads 16-byte chunks with buffer loads
-//     (the buffer range is the operand, so rows past M / N read as zero and
-//     no edge code is needed in the main loop);
-//   * VGPR -> LDS: double-buffered stages, rows of BK elements without
-//     padding; the 16-byte chunk c of row r is stored at slot c ^ swz(r), so
-//     both the stores (8 lanes fill one 128-byte row segment) and the MFMA
-//     fragment reads (8 lanes read 8 different rows) hit 8 distinct 16-byte
-//     bank groups;
-//   * LDS -> MFMA: fragments of the next 16-k group are read while the MFMAs
-//     of the current group run; the stage barrier sits before the last group
-//     so the first fragments of the next stage are in flight during it.
-// One barrier per K step; the global loads for step k+2 are issued at the
-// start of step k.
+// lane).  K advances in steps of BK (16 for two-part operands, 32 for BF16),
+// so one stage (every part of both operand tiles for one step) is 32 KB.
+//   * global -> LDS: buffer loads straight into LDS (no VGPR staging) fill a
+//     ring of 4 stages, three steps ahead of the MFMAs (about 2 us of memory
+//     latency hidden).  The buffer range is the operand, so rows past M / N
+//     read as zero and no edge code is needed in the main loop.  Each wave
+//     fills 8 x 1 KB of a stage per step (one part of one operand for the
+//     split formats).
+//   * layout: rows of BK elements without padding; the 16-byte chunk c of
+//     row r sits at slot c ^ swz(r).  A DMA instruction writes 1 KB of
+//     consecutive slots, so the swizzle is applied by choosing the global
+//     chunk each lane fetches; MFMA fragment reads (16 lanes on 16 rows at
+//     one logical chunk) then hit 16 different bank groups.
+//   * LDS -> MFMA: the fragments of the next 16-k group are read while the
+//     MFMAs of the current group run.
+// One barrier per K step; it waits only for this wave's oldest in-flight
+// stage, never for the loads still in flight behind it.
 #pragma once
+
+#include <type_traits>
 
 #include "gemm_core.hpp"
 
@@ -36,7 +43,7 @@ typedef int i32x4 __attribute__((ext_vector_type(4)));
 constexpr int kBM = 256, kBN = 256;
 
 struct Problem {
-  int M, N, kp;          // kp: packed row pitch (elements), multiple of BK
+  int M, N, kp;          // kp: padded K (elements), a multiple of 4 K steps
   float alpha, beta;
   const float* alpha_dev;  // optional device factor multiplied into alpha
   const uint16_t* a[2];    // hi, lo parts of op(A): [batch][M][kp]
@@ -52,23 +59,36 @@ struct Problem {
 template <int BK, int TERMS>
 struct Cfg {
   static constexpr int NPARTS = TERMS == 1 ? 1 : 2;
-  static constexpr int CH = BK / 8;                // 16-byte chunks per row
+  static constexpr int PARTS = 2 * NPARTS;            // A parts, then B parts
+  static constexpr int CH = BK / 8;                   // 16-byte chunks per row
   static constexpr int ROW_BYTES = BK * 2;
   static constexpr int TILE_BYTES = kBM * ROW_BYTES;  // one part of one operand
-  static constexpr int LOADS = TILE_BYTES / (16 * 256);  // chunks per thread per part
-  static constexpr int ROWS_PER_LOAD = 256 / CH;   // rows covered by one chunk index
-  static constexpr int STAGE_BYTES = 2 * NPARTS * TILE_BYTES;
-  static constexpr int LDS_BYTES = 2 * STAGE_BYTES;
+  static constexpr int STAGE_BYTES = PARTS * TILE_BYTES;
+  static constexpr int NSTAGE = 4;
   static constexpr int GROUPS = BK / 16;
-  static_assert(BK == 32 || BK == 64, "BK");
-  static_assert(LDS_BYTES <= 160 * 1024, "LDS");
+  static constexpr int ROWS_PER_DMA = 1024 / ROW_BYTES;  // rows one wave DMA fills
+  static constexpr int DMA_PER_PART = kBM / ROWS_PER_DMA;
+  static constexpr int DMA_PER_WAVE = PARTS * DMA_PER_PART / 4;
+  static_assert((BK == 16 && TERMS > 1) || (BK == 32 && TERMS == 1), "BK");
+  static_assert(STAGE_BYTES == 32 * 1024 && DMA_PER_WAVE == 8, "stage shape");
+  static_assert(NSTAGE * STAGE_BYTES <= 160 * 1024, "LDS");
 };
 
-// Slot permutation: rows that share a 128-byte bank line set get different
-// chunk slots.
+// K steps per ring revolution; the packed pitch must be a multiple of this
+// many elements.
+template <int BK> constexpr int kKAlign = 4 * BK;
+
+// Slot permutation of a row's 16-byte chunks.  A fragment read has 16
+// consecutive lanes on 16 consecutive rows at one logical chunk, a stage
+// store has consecutive lanes along consecutive chunks; with the slot
+// chunk ^ swz(row) both land on 16 different 16-byte bank groups of every
+// 256-byte window (and 8 of every 128-byte one).  Depends on row bits 0-4
+// only, so it is a per-lane constant for fragment rows w + 32 i + lane % 32.
 template <int CH>
 __device__ __forceinline__ int swz(int row) {
-  return (row / (8 / CH)) % CH;
+  if constexpr (CH == 2) return ((row >> 2) ^ (row >> 3)) & 1;
+  else if constexpr (CH == 4) return ((row >> 1) ^ (row >> 3)) & 3;
+  else return (row ^ ((row >> 3) & 3)) & 7;
 }
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, uint32_t bytes) {
@@ -77,15 +97,15 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* base, ui
                                            0x00020000);
 }
 
-// LDS-only workgroup barrier: waits for this wave's LDS operations, not for
-// its outstanding global loads (those land in registers two steps ahead).
-__device__ __forceinline__ void lds_barrier() {
-  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+// Stage barrier: this wave's DMA into the next stage to be read has landed
+// (at most 2 x 8 of its LDS loads, the two younger stages, still in flight)
+// and its LDS reads are done; then the workgroup barrier.
+__device__ __forceinline__ void stage_barrier() {
+  asm volatile("s_waitcnt vmcnt(16) lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
 // Scheduling: the next N instructions of class MASK (0x020 global load,
-// 0x100 LDS read, 0x200 LDS write) go PER at a time between consecutive
-// MFMAs.
+// 0x100 LDS read) go PER at a time between consecutive MFMAs.
 template <int N, int MASK, int PER>
 __device__ __forceinline__ void interleave() {
   static_assert(N % PER == 0, "uneven interleave");
@@ -97,19 +117,35 @@ __device__ __forceinline__ void interleave() {
 }
 
 template <int TERMS> constexpr int kFragReads = TERMS == 1 ? 8 : 16;
-template <int BK, int TERMS>
-constexpr int kStoresPerStep = 2 * (TERMS == 1 ? 1 : 2) * (256 * BK * 2 / (16 * 256));
+
+typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
 template <int BK, int TERMS>
 __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
+  // The body is device code only (address-space casts and LDS loads have no
+  // host meaning); the host pass needs just the launch stub.
+#if defined(__HIP_DEVICE_COMPILE__)
   using C = Cfg<BK, TERMS>;
   constexpr int NP = C::NPARTS;
   constexpr int G = C::GROUPS;
-  __shared__ __attribute__((aligned(16))) uint8_t lds[C::LDS_BYTES];
+  // One array per ring stage: the compiler then tells the LDS loads of one
+  // stage from the reads of another.
+  __shared__ __attribute__((aligned(1024))) uint8_t st0[C::STAGE_BYTES];
+  __shared__ __attribute__((aligned(1024))) uint8_t st1[C::STAGE_BYTES];
+  __shared__ __attribute__((aligned(1024))) uint8_t st2[C::STAGE_BYTES];
+  __shared__ __attribute__((aligned(1024))) uint8_t st3[C::STAGE_BYTES];
+  auto stage = [&](auto r) -> uint8_t* {
+    if constexpr (decltype(r)::value == 0) return st0;
+    else if constexpr (decltype(r)::value == 1) return st1;
+    else if constexpr (decltype(r)::value == 2) return st2;
+    else return st3;
+  };
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
-  const int wave = tid >> 6;
+  // Wave-uniform values go through readfirstlane so that the buffer
+  // descriptor and the LDS destination stay in scalar registers.
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = (wave & 1) * 128, wn = (wave >> 1) * 128;
 
   int64_t tm64, tn64;
@@ -117,71 +153,65 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   const int m0 = int(tm64) * kBM, n0 = int(tn64) * kBN;
   const int bidx = blockIdx.y;
 
-  // Buffer resources: one per part per operand; the range is the operand.
-  __amdgpu_buffer_rsrc_t ra[NP], rb[NP];
-  const uint32_t a_bytes = uint32_t(p.M) * uint32_t(p.kp) * 2u;
-  const uint32_t b_bytes = uint32_t(p.N) * uint32_t(p.kp) * 2u;
+  // This wave's DMA work per step: 8 x 1 KB of part `part` (A parts first),
+  // row blocks rb0 .. rb0 + 7.
+  const int q0 = wave * C::DMA_PER_WAVE;
+  const int part = q0 / C::DMA_PER_PART;
+  const int rb0 = q0 % C::DMA_PER_PART;
+  const bool is_a = part < NP;
+  const int pidx = is_a ? part : part - NP;
+  const uint16_t* base = is_a ? p.a[pidx] + int64_t(bidx) * p.a_stride
+                              : p.b[pidx] + int64_t(bidx) * p.b_stride;
+  const uint32_t rows = uint32_t(is_a ? p.M : p.N);
+  // Packed operands are K-slab-major, [kp / BK][rows][BK]: step kt of rows
+  // r .. r + ROWS_PER_DMA - 1 is one contiguous kilobyte.  Each step gets its
+  // own buffer descriptor whose range is that slab, so rows past the
+  // operand's end read as zero.
+  const uint32_t slab_bytes = rows * uint32_t(C::ROW_BYTES);
+  const int row0 = is_a ? m0 : n0;
+  // Lane l of DMA j writes slot l % CH of tile row (rb0 + j) * ROWS_PER_DMA +
+  // l / CH, so it fetches chunk (l % CH) ^ swz(row) of that row.
+  uint32_t voff[C::DMA_PER_WAVE];
 #pragma unroll
-  for (int t = 0; t < NP; ++t) {
-    ra[t] = make_rsrc(p.a[t] + int64_t(bidx) * p.a_stride, a_bytes);
-    rb[t] = make_rsrc(p.b[t] + int64_t(bidx) * p.b_stride, b_bytes);
+  for (int j = 0; j < C::DMA_PER_WAVE; ++j) {
+    const int r = (rb0 + j) * C::ROWS_PER_DMA + lane / C::CH;
+    voff[j] = uint32_t(row0 + r) * uint32_t(C::ROW_BYTES) +
+              uint32_t((lane % C::CH) ^ swz<C::CH>(r)) * 16u;
   }
-  // This thread's chunks: row tid / CH + ROWS_PER_LOAD * i, chunk tid % CH.
-  const int lrow = tid / C::CH, lkc = tid % C::CH;
-  const uint32_t pitch = uint32_t(p.kp) * 2u;
-  const uint32_t voff_a = uint32_t(m0 + lrow) * pitch + uint32_t(lkc) * 16u;
-  const uint32_t voff_b = uint32_t(n0 + lrow) * pitch + uint32_t(lkc) * 16u;
-  const uint32_t vstep = uint32_t(C::ROWS_PER_LOAD) * pitch;
-  // LDS byte offset of this thread's chunk i within a part tile.
-  const int st_off = lrow * C::ROW_BYTES + ((lkc ^ swz<C::CH>(lrow)) * 16);
+  const int dma_lds = part * C::TILE_BYTES + rb0 * 1024;
 
-  // Staging registers: [operand part][chunk]; parts 0..NP-1 = A, NP.. = B.
-  i32x4 stg[2 * NP][C::LOADS];
-
-  auto gload = [&](int k0) {
-    const int soff = k0 * 2;
+  const int nk = p.kp / BK;
+  auto dma = [&](auto r, int kt) {
+    const int ks = kt < nk ? kt : nk - 1;  // past the end: re-read, never consumed
+    const __amdgpu_buffer_rsrc_t rsrc =
+        make_rsrc(base + int64_t(ks) * rows * BK, slab_bytes);
+    uint8_t* dst = stage(r) + dma_lds;
 #pragma unroll
-    for (int t = 0; t < NP; ++t)
-#pragma unroll
-      for (int i = 0; i < C::LOADS; ++i) {
-        stg[t][i] = __builtin_amdgcn_raw_buffer_load_b128(ra[t], voff_a + uint32_t(i) * vstep,
-                                                          soff, 0);
-        stg[NP + t][i] = __builtin_amdgcn_raw_buffer_load_b128(
-            rb[t], voff_b + uint32_t(i) * vstep, soff, 0);
-      }
-  };
-  // Part tile order inside a stage: A parts, then B parts.
-  auto lstore = [&](int stage) {
-    uint8_t* base = lds + stage * C::STAGE_BYTES + st_off;
-#pragma unroll
-    for (int t = 0; t < 2 * NP; ++t)
-#pragma unroll
-      for (int i = 0; i < C::LOADS; ++i)
-        *reinterpret_cast<i32x4*>(base + t * C::TILE_BYTES +
-                                  i * C::ROWS_PER_LOAD * C::ROW_BYTES) = stg[t][i];
+    for (int j = 0; j < C::DMA_PER_WAVE; ++j)
+      __builtin_amdgcn_raw_ptr_buffer_load_lds(rsrc, (lds_ptr_t)(dst + j * 1024), 16, voff[j], 0,
+                                               0, 0);
   };
 
-  // Fragment read offsets (bytes within a part tile) for group g: row
-  // w + 32 i + lane % 32, chunk 2 g + lane / 32.
+  // Fragment reads: row w + 32 i + lane % 32 of a part tile, chunk 2 g +
+  // lane / 32, at slot chunk ^ swz(row) (swz depends on lane % 32 only).
   const int frow = lane & 31;
   const int fh = lane >> 5;
-  const int fswz = swz<C::CH>(frow);  // (w + 32 i) is a multiple of 8
+  const int fswz = swz<C::CH>(frow);
   int foff[G];
 #pragma unroll
   for (int g = 0; g < G; ++g) foff[g] = frow * C::ROW_BYTES + (((2 * g + fh) ^ fswz) * 16);
 
-  // Fragments: [buffer][part][i] for A and B.
   bf16x8 fa[2][NP][4], fb[2][NP][4];
-  auto fread = [&](int stage, int g, int buf) {
-    const uint8_t* base = lds + stage * C::STAGE_BYTES + foff[g];
+  auto fread = [&](auto r, int g, int buf) {
+    const uint8_t* b = stage(r) + foff[g];
 #pragma unroll
     for (int t = 0; t < NP; ++t)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         fa[buf][t][i] = *reinterpret_cast<const bf16x8*>(
-            base + t * C::TILE_BYTES + (wm + 32 * i) * C::ROW_BYTES);
+            b + t * C::TILE_BYTES + (wm + 32 * i) * C::ROW_BYTES);
         fb[buf][t][i] = *reinterpret_cast<const bf16x8*>(
-            base + (NP + t) * C::TILE_BYTES + (wn + 32 * i) * C::ROW_BYTES);
+            b + (NP + t) * C::TILE_BYTES + (wn + 32 * i) * C::ROW_BYTES);
       }
   };
 
@@ -226,60 +256,57 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
                                                             acc[i][j], 0, 0, 0);
   };
 
-  const int nk = p.kp / BK;
-  // The loop body is branch-free (so the interleaving below applies to all of
-  // it): the loads past the last step re-read the last K tile and the stores
-  // and fragment reads they feed are never consumed.
-  auto kofs = [&](int kt) { return (kt < nk ? kt : nk - 1) * BK; };
-  gload(0);
-  lstore(0);
-  gload(kofs(1));
-  lds_barrier();
-  fread(0, 0, 0);
-
-  constexpr int NST = kStoresPerStep<BK, TERMS>;  // LDS stores = global loads per step
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
   constexpr int NFR = kFragReads<TERMS>;
-  for (int kt = 0; kt < nk; ++kt) {
-    const int s = kt & 1;
-    // Groups before the last: the next stage's LDS stores, the global loads
-    // two steps ahead and the fragment reads of the following group are
-    // spread between the MFMAs.
-    if constexpr (G == 2) {
-      lstore(s ^ 1);
-      gload(kofs(kt + 2));
-      fread(s, 1, 1);
-      mma(0);
-      interleave<NST, 0x200, 1>();
-      interleave<NST, 0x020, 1>();
+
+  // Prologue: steps 0..3 in flight; wait for step 0 and read its group 0.
+  dma(I0{}, 0);
+  dma(I1{}, 1);
+  dma(I2{}, 2);
+  dma(I3{}, 3);
+  asm volatile("s_waitcnt vmcnt(24)\n\ts_barrier" ::: "memory");
+  fread(I0{}, 0, 0);
+
+  // Step kt lives in stage kt % 4.  Per step: wait for step kt + 1, refill
+  // the stage step kt's fragments came from with step kt + 4 (G == 1), and
+  // run step kt's MFMAs while the next group's fragments are read.
+  auto step = [&](auto r, int kt) {
+    constexpr int R = decltype(r)::value;
+    using RN = std::integral_constant<int, (R + 1) % 4>;
+    if constexpr (G == 1) {
+      stage_barrier();
+      dma(r, kt + 4);
+      fread(RN{}, 0, (R + 1) & 1);
+      mma(R & 1);
+      interleave<C::DMA_PER_WAVE, 0x020, 1>();
       interleave<NFR, 0x100, 1>();
       __builtin_amdgcn_sched_barrier(0);
     } else {
-      lstore(s ^ 1);
-      fread(s, 1, 1);
+      fread(r, 1, 1);
       mma(0);
-      interleave<NST, 0x200, 2>();
       interleave<NFR, 0x100, 1>();
       __builtin_amdgcn_sched_barrier(0);
-      gload(kofs(kt + 2));
-      fread(s, 2, 0);
+      stage_barrier();
+      dma(r, kt + 4);
+      fread(RN{}, 0, 0);
       mma(1);
-      interleave<NST, 0x020, 2>();
-      interleave<NFR, 0x100, 1>();
-      __builtin_amdgcn_sched_barrier(0);
-      fread(s, 3, 1);
-      mma(0);
+      interleave<C::DMA_PER_WAVE, 0x020, 1>();
       interleave<NFR, 0x100, 1>();
       __builtin_amdgcn_sched_barrier(0);
     }
-    // Every wave has stored the next stage and read its last fragments of
-    // this one: swap stages, then fetch the next step's first group while
-    // the last group's MFMAs run.
-    lds_barrier();
-    fread(s ^ 1, 0, 0);
-    mma(1);
-    interleave<NFR, 0x100, 1>();
-    __builtin_amdgcn_sched_barrier(0);
+  };
+  for (int kt = 0; kt < nk; kt += 4) {
+    step(I0{}, kt);
+    step(I1{}, kt + 1);
+    step(I2{}, kt + 2);
+    step(I3{}, kt + 3);
   }
+  // The ring's last loads (re-reads past the end) must land before the
+  // workgroup's LDS can be handed to another workgroup.
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
 
   // Epilogue: C = alpha * acc + beta * C.  Lane holds column wn + 32 j +
   // lane % 32 and rows wm + 32 i + 8 (e / 4) + 4 (lane / 32) + e % 4.
@@ -318,19 +345,22 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
         }
       }
   }
+#endif
 }
 
-// True when the packed operands fit the kernel's 32-bit buffer offsets.
+// True when the kernel's 32-bit buffer ranges and offsets hold (one slab
+// of up to 64 bytes per row, plus the tile overhang).
 inline bool fits(int64_t M, int64_t N, int64_t kp) {
+  (void)kp;
   const int64_t lim = int64_t(1) << 31;
-  return (M + kBM) * kp * 2 < lim && (N + kBN) * kp * 2 < lim;
+  return (M + kBM) * 64 < lim && (N + kBN) * 64 < lim;
 }
 
 template <int BK, int TERMS>
 pb_status_t launch(pb_handle_t h, Problem p) {
   p.tiles_m = int(pb::ceil_div(p.M, kBM));
   p.tiles_n = int(pb::ceil_div(p.N, kBN));
-  PB_REQUIRE(p.kp % BK == 0, "packed pitch must be a multiple of %d", BK);
+  PB_REQUIRE(p.kp % kKAlign<BK> == 0, "packed pitch must be a multiple of %d", kKAlign<BK>);
   PB_REQUIRE(p.batch >= 1 && p.batch <= 65535, "batch out of range");
   const dim3 grid{unsigned(p.tiles_m) * unsigned(p.tiles_n), unsigned(p.batch)};
   hipLaunchKernelGGL((mt_kernel<BK, TERMS>), grid, dim3(256), 0, h->stream, p);

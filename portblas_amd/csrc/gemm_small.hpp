@@ -1,0 +1,179 @@
+// Many small GEMMs (M, N <= 64) of a strided batch: one problem per
+// workgroup.  The batched path of portBLAS runs its tiled kernel over the
+// batch (src/operations/blas3/gemm_local.hpp:273-279,511-516); a 64 x 64
+// problem is one tile, so what matters is latency: every thread issues all of
+// its global loads for a K chunk of 64 at once, the chunk goes through LDS
+// ([k][m] and [k][n], row pitch 80 elements so MFMA fragment reads are
+// conflict-free), and four waves each compute a 32 x 32 quarter of C with
+// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f64_16x16x4_f64 (FP64).  Small
+// LDS and register footprints let 4 (FP32) or 2 (FP64) workgroups share a
+// CU.
+#pragma once
+
+#include "gemm_launch.hpp"
+
+namespace pbgemm {
+namespace small {
+
+constexpr int kTile = 64;   // M, N (and the K chunk) of one problem tile
+constexpr int kPitch = 80;  // LDS row pitch (elements)
+
+// Vector loads need 16-byte alignment of every element group they touch.
+template <typename T>
+__device__ __forceinline__ void load_row_vec(const T* p, T out[16 / sizeof(T)]) {
+  if constexpr (sizeof(T) == 4) {
+    const pbdev::f32x4_t v = *reinterpret_cast<const pbdev::f32x4_t*>(p);
+    out[0] = v.x; out[1] = v.y; out[2] = v.z; out[3] = v.w;
+  } else {
+    const pbdev::f64x2_t v = *reinterpret_cast<const pbdev::f64x2_t*>(p);
+    out[0] = v.x; out[1] = v.y;
+  }
+}
+
+// One K chunk of op(X) (element (r, k) at x[r*rs + k*ks], zero outside
+// rows x K) for LDS as dst[k][r]: load() issues every global load of the
+// thread, store() writes them to LDS.  VEC: 16-byte vectors along the
+// contiguous dimension (rs == 1 or ks == 1; aligned, with a pitch that is a
+// multiple of the vector width); otherwise scalar loads.
+template <typename T, bool VEC>
+struct Chunk {
+  static constexpr int VN = 16 / sizeof(T);
+  static constexpr int IT = VEC ? kTile * kTile / VN / 256 : kTile * kTile / 256;
+  static constexpr int W = VEC ? VN : 1;
+  T v[IT][W];
+
+  __device__ __forceinline__ void load(const T* x, int64_t rs, int64_t ks, int rows, int64_t K,
+                                       int64_t k0) {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + 256 * it;
+      if (VEC && rs == 1) {  // vectors along r: (r .. r+VN-1, k)
+        const int r = (e % (kTile / VN)) * VN, k = e / (kTile / VN);
+        if (k0 + k < K && r + VN <= rows) {
+          load_row_vec<T>(x + r + (k0 + k) * ks, v[it]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < W; ++i)
+            v[it][i] = (k0 + k < K && r + i < rows) ? x[(r + i) + (k0 + k) * ks] : T(0);
+        }
+      } else if (VEC) {  // ks == 1, vectors along k: (r, k .. k+VN-1)
+        const int k = (e % (kTile / VN)) * VN, r = e / (kTile / VN);
+        if (r < rows && k0 + k + VN <= K) {
+          load_row_vec<T>(x + r * rs + (k0 + k), v[it]);
+        } else {
+#pragma unroll
+          for (int i = 0; i < W; ++i)
+            v[it][i] = (r < rows && k0 + k + i < K) ? x[r * rs + (k0 + k + i)] : T(0);
+        }
+      } else {
+        const int r = e % kTile, k = e / kTile;
+        v[it][0] = (r < rows && k0 + k < K) ? x[r * rs + (k0 + k) * ks] : T(0);
+      }
+    }
+  }
+
+  __device__ __forceinline__ void store(int64_t rs, T (*dst)[kPitch]) const {
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+      const int e = threadIdx.x + 256 * it;
+      if (VEC && rs == 1) {
+        const int r = (e % (kTile / VN)) * VN, k = e / (kTile / VN);
+        if constexpr (sizeof(T) == 4)
+          *reinterpret_cast<pbdev::f32x4_t*>(&dst[k][r]) =
+              pbdev::f32x4_t{v[it][0], v[it][1], v[it][2], v[it][3]};
+        else
+          *reinterpret_cast<pbdev::f64x2_t*>(&dst[k][r]) = pbdev::f64x2_t{v[it][0], v[it][1]};
+      } else if (VEC) {
+        const int k = (e % (kTile / VN)) * VN, r = e / (kTile / VN);
+#pragma unroll
+        for (int i = 0; i < W; ++i) dst[k + i][r] = v[it][i];
+      } else {
+        dst[e / kTile][e % kTile] = v[it][0];
+      }
+    }
+  }
+};
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
+  using Acc = typename Traits<T>::AccV;  // 4 accumulators per lane (f32x4 / f64x4)
+  __shared__ __attribute__((aligned(16))) T As[kTile][kPitch];
+  __shared__ __attribute__((aligned(16))) T Bs[kTile][kPitch];
+  const int64_t b = blockIdx.x;
+  const T* A = p.a[0] + b * p.a_stride;
+  const T* B = p.b[0] + b * p.b_stride;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  const int M = int(p.M), N = int(p.N);
+  Acc acc[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
+  for (int64_t k0 = 0; k0 < p.K; k0 += kTile) {
+    Chunk<T, VEC> ca, cb;
+    ca.load(A, p.a_rs, p.a_ks, M, p.K, k0);
+    cb.load(B, p.b_rs, p.b_ks, N, p.K, k0);
+    if (k0 > 0) __syncthreads();  // previous chunk fully consumed
+    ca.store(p.a_rs, As);
+    cb.store(p.b_rs, Bs);
+    __syncthreads();
+#pragma unroll 4
+    for (int k = 0; k < kTile; k += 4) {
+      const int kk = k + lane / 16;
+      T a[2], bv[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) a[i] = As[kk][wm + 16 * i + lane % 16];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk][wn + 16 * j + lane % 16];
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          if constexpr (sizeof(T) == 4)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], bv[j], acc[i][j], 0, 0, 0);
+          else
+            acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[i], bv[j], acc[i][j], 0, 0, 0);
+        }
+    }
+  }
+  // C = alpha * acc + beta * C (beta == 0: C not read).  Accumulator e of a
+  // lane holds column lane % 16 and row acc_row(lane, e).
+  T* C = p.c + b * p.c_stride;
+  const T alpha = p.alpha, beta = p.beta;
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int n = wn + 16 * j + lane % 16;
+      if (n >= N) continue;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int m = wm + 16 * i + acc_row<T>(lane, e);
+        if (m >= M) continue;
+        T* cp = C + m * p.c_rs + int64_t(n) * p.ldc;
+        const T v = alpha * acc[i][j][e];
+        *cp = (beta != T(0)) ? v + beta * *cp : v;
+      }
+    }
+}
+
+// One problem per workgroup pays off for a batch of at least one problem
+// per CU with M, N <= 64 (interleaved batches keep the tile engine).
+inline bool eligible(pb_handle_t h, int64_t M, int64_t N, int64_t batch, bool interleaved) {
+  return !interleaved && M <= kTile && N <= kTile && batch >= h->num_cus && batch <= 0x7fffffff;
+}
+
+template <typename T>
+pb_status_t launch(pb_handle_t h, const GemmProblem<T>& p, bool vec) {
+  const dim3 grid{unsigned(p.batch)};
+  if (vec) hipLaunchKernelGGL((small_gemm_kernel<T, true>), grid, dim3(256), 0, h->stream, p);
+  else hipLaunchKernelGGL((small_gemm_kernel<T, false>), grid, dim3(256), 0, h->stream, p);
+  PB_CHECK_LAUNCH();
+  return PB_STATUS_SUCCESS;
+}
+
+}  // namespace small
+}  // namespace pbgemm

@@ -7,9 +7,9 @@
 // y = beta*y + alpha*sum (beta == 0 -> y not read, :185-212).
 //
 // MI355X design (HBM-bound; A is read once with non-temporal 16-byte loads):
-//  'n'  a workgroup of 8 waves owns 256 consecutive rows; every lane holds 4
-//       consecutive rows (one 16-byte load per column for float) and the 8
-//       waves split the workgroup's column chunk.  Partial sums are folded
+//  'n'  a workgroup of 8 waves owns 256 or 512 consecutive rows; every lane
+//       holds 4 consecutive rows per 256 (one 16-byte load per column for
+//       float) and the 8 waves split the workgroup's column chunk.  Partial sums are folded
 //       across waves in LDS; with several column chunks, each writes its
 //       partials to the workspace and a fold kernel sums them in a fixed
 //       order and applies the epilogue.
@@ -51,68 +51,86 @@ __device__ __forceinline__ void store_y(T* y, int64_t idx, T alpha, T beta, T su
 
 // ---------------------------------------------------------------------------
 // 'n': grid (row_tiles, col_chunks), kNBlock threads (kNWaves waves).  A
-// workgroup owns kNRows (256) consecutive rows; every lane holds 4 of them
-// (rows r0 + 4 lane + e: one 16-byte load per column for float, VEC = A
-// 16-byte aligned with lda a multiple of 4 elements; scalar: r0 + 64 e +
-// lane).  The waves interleave the chunk's columns, kNU columns per step, so
+// workgroup owns 256 Q consecutive rows; every lane holds 4 of every 256
+// (rows r0 + 256 q + 4 lane + e: one 16-byte load per column for float, VEC
+// = A 16-byte aligned with lda a multiple of 4 elements; scalar: r0 + 256 q
+// + 64 e + lane).  The waves interleave the chunk's columns, kNU columns per step, so
 // each wave keeps kNU 1 KB column segments in flight.  The waves' sums are
 // folded in LDS in a fixed order.  Eight waves per workgroup keep the number
 // of column chunks (and so the fold traffic) small.
 // ---------------------------------------------------------------------------
 constexpr int kNWaves = 8;
 constexpr int kNBlock = 64 * kNWaves;
-constexpr int kNRows = 256;
 constexpr int kNU = 8;
 
-template <typename T, bool VEC>
+// Q: 256-row groups per workgroup (each lane holds 4 Q rows; with Q > 1 a
+// wave reads Q KB of every column).
+template <typename T, bool VEC, int Q>
 __global__ __launch_bounds__(kNBlock) void gemv_n_kernel(
     int64_t M, int64_t N, T alpha, const T* __restrict__ A, int64_t lda,
     const T* __restrict__ x, int64_t incx, T beta, T* y, int64_t incy,
     int64_t cols_per_chunk, T* __restrict__ partials) {
-  __shared__ T red[kNWaves][kNRows];
+  constexpr int ROWS = 256 * Q;
+  constexpr int U = kNU / Q;  // columns per step: U x Q 16-byte loads in flight
+  __shared__ T red[kNWaves][ROWS];
   const int lane = threadIdx.x & 63;
   const int wave = threadIdx.x >> 6;
-  const int64_t r0 = int64_t(blockIdx.x) * kNRows;
+  const int64_t r0 = int64_t(blockIdx.x) * ROWS;
   const int64_t c0 = int64_t(blockIdx.y) * cols_per_chunk;
   int64_t c1 = c0 + cols_per_chunk;
   if (c1 > N) c1 = N;
-  T acc[4] = {T(0), T(0), T(0), T(0)};
-  if (VEC && r0 + kNRows <= M) {
+  T acc[Q][4];
+#pragma unroll
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) acc[q][e] = T(0);
+  if (VEC && r0 + ROWS <= M) {
     const T* Ap = A + r0 + 4 * lane;
     int64_t j = c0 + wave;
-    for (; j + (kNU - 1) * kNWaves < c1; j += kNU * kNWaves) {
-      Four<T> a[kNU];
-      T xs[kNU];
+    for (; j + (U - 1) * kNWaves < c1; j += U * kNWaves) {
+      Four<T> a[U][Q];
+      T xs[U];
 #pragma unroll
-      for (int u = 0; u < kNU; ++u) a[u] = load4(Ap + (j + u * kNWaves) * lda);
+      for (int u = 0; u < U; ++u)
 #pragma unroll
-      for (int u = 0; u < kNU; ++u) xs[u] = x[pbdev::vidx(j + u * kNWaves, N, incx)];
+        for (int q = 0; q < Q; ++q) a[u][q] = load4(Ap + (j + u * kNWaves) * lda + 256 * q);
 #pragma unroll
-      for (int u = 0; u < kNU; ++u)
+      for (int u = 0; u < U; ++u) xs[u] = x[pbdev::vidx(j + u * kNWaves, N, incx)];
 #pragma unroll
-        for (int e = 0; e < 4; ++e) acc[e] += a[u].v[e] * xs[u];
+      for (int u = 0; u < U; ++u)
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc[q][e] += a[u][q].v[e] * xs[u];
     }
     for (; j < c1; j += kNWaves) {
-      const Four<T> a = load4(Ap + j * lda);
       const T xj = x[pbdev::vidx(j, N, incx)];
 #pragma unroll
-      for (int e = 0; e < 4; ++e) acc[e] += a.v[e] * xj;
+      for (int q = 0; q < Q; ++q) {
+        const Four<T> a = load4(Ap + j * lda + 256 * q);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) acc[q][e] += a.v[e] * xj;
+      }
     }
   } else {
     for (int64_t j = c0 + wave; j < c1; j += kNWaves) {
       const T xj = x[pbdev::vidx(j, N, incx)];
       const T* col = A + j * lda;
 #pragma unroll
-      for (int e = 0; e < 4; ++e) {
-        const int64_t r = VEC ? r0 + 4 * lane + e : r0 + 64 * e + lane;
-        if (r < M) acc[e] += pbdev::ld_stream(col + r) * xj;
-      }
+      for (int q = 0; q < Q; ++q)
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t r = VEC ? r0 + 256 * q + 4 * lane + e : r0 + 256 * q + 64 * e + lane;
+          if (r < M) acc[q][e] += pbdev::ld_stream(col + r) * xj;
+        }
     }
   }
 #pragma unroll
-  for (int e = 0; e < 4; ++e) red[wave][VEC ? 4 * lane + e : 64 * e + lane] = acc[e];
+  for (int q = 0; q < Q; ++q)
+#pragma unroll
+    for (int e = 0; e < 4; ++e) red[wave][256 * q + (VEC ? 4 * lane + e : 64 * e + lane)] = acc[q][e];
   __syncthreads();
-  for (int rr = threadIdx.x; rr < kNRows; rr += kNBlock) {
+  for (int rr = threadIdx.x; rr < ROWS; rr += kNBlock) {
     T s = red[0][rr];
 #pragma unroll
     for (int w = 1; w < kNWaves; ++w) s += red[w][rr];
@@ -271,8 +289,11 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
   const bool avec = aligned16(A) && (lda * int64_t(sizeof(T))) % 16 == 0;
   const int target = 4 * h->num_cus;
   if (!tr) {
-    // About two 8-wave workgroups per CU; every chunk keeps >= 8 columns per wave.
-    const int64_t tiles = pb::ceil_div(M, kNRows);
+    // About two 8-wave workgroups per CU; every chunk keeps >= 8 columns per
+    // wave.  Tall matrices take 512-row tiles (2 KB of every column per wave).
+    const int q = M >= 4096 ? 2 : 1;
+    const int64_t rows = 256 * q;
+    const int64_t tiles = pb::ceil_div(M, rows);
     PB_REQUIRE(tiles <= 0x7fffffffLL, "M too large");
     int64_t chunks = pb::ceil_div(2 * h->num_cus, tiles);
     const int64_t max_chunks = pb::ceil_div(N, 8 * kNWaves);
@@ -285,12 +306,14 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
     if (chunks > 1) PB_TRY(pb::workspace(h, sizeof(T) * size_t(chunks) * size_t(M), &ws));
     T* part = static_cast<T*>(ws);
     dim3 grid{unsigned(tiles), unsigned(chunks)};
-    if (avec)
-      hipLaunchKernelGGL((gemv_n_kernel<T, true>), grid, dim3(kNBlock), 0, h->stream, M, N,
-                         alpha, A, lda, x, incx, beta, y, incy, cpc, part);
-    else
-      hipLaunchKernelGGL((gemv_n_kernel<T, false>), grid, dim3(kNBlock), 0, h->stream, M, N,
-                         alpha, A, lda, x, incx, beta, y, incy, cpc, part);
+#define PB_GEMV_N(V, Q)                                                                      \
+  hipLaunchKernelGGL((gemv_n_kernel<T, V, Q>), grid, dim3(kNBlock), 0, h->stream, M, N, alpha, A, \
+                     lda, x, incx, beta, y, incy, cpc, part)
+    if (avec && q == 2) PB_GEMV_N(true, 2);
+    else if (avec) PB_GEMV_N(true, 1);
+    else if (q == 2) PB_GEMV_N(false, 2);
+    else PB_GEMV_N(false, 1);
+#undef PB_GEMV_N
     if (chunks > 1)
       hipLaunchKernelGGL(gemv_fold_kernel<T>, dim3(unsigned(pb::ceil_div(M, kBlock))),
                          dim3(kBlock), 0, h->stream, M, int(chunks), part, alpha, beta, y, incy);

@@ -79,8 +79,150 @@ __global__ __launch_bounds__(256) void grouped_kernel(const DevProblem* __restri
   }
 }
 
+// BF16 problems: a lean 64 x 64 tile.  18 KB of LDS and few registers let
+// about 8 workgroups share a CU, so the latency of one tile's loads is
+// hidden by the other resident tiles rather than by deep pipelining (most
+// grouped problems are a handful of tiles and a few K chunks).  Per K chunk
+// of 64 every thread issues its 2 + 2 16-byte loads, then writes them to LDS
+// K-contiguous ([row][k], pitch 72): a K-contiguous operand with one
+// 16-byte store, an M/N-contiguous one transposed with eight 2-byte stores.
+// Each wave computes a 32 x 32 quarter with v_mfma_f32_32x32x16_bf16.
+constexpr int kLT = 64, kLP = 72;
+
+template <bool KC>
+__device__ __forceinline__ void lean_load(const uint16_t* x, int64_t rs, int64_t ks, int64_t rows,
+                                          int64_t K, int64_t r0, int64_t k0, bool vec, int e,
+                                          u32x4& out) {
+  // e in [0, 512): KC -> row e / 8, k group (e % 8) * 8; else k e / 8, row group (e % 8) * 8.
+  const int64_t r = KC ? r0 + e / 8 : r0 + (e % 8) * 8;
+  const int64_t k = KC ? k0 + (e % 8) * 8 : k0 + e / 8;
+  if (vec && (KC ? (r < rows && k + 8 <= K) : (k < K && r + 8 <= rows))) {
+    out = *reinterpret_cast<const u32x4*>(x + r * rs + k * ks);
+    return;
+  }
+  // Edge or unaligned: element by element (kept rolled: it is rare, and
+  // unrolled it would cost the registers that keep 3+ workgroups per CU).
+  uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll 1
+  for (int i = 0; i < 8; ++i) {
+    const int64_t ri = KC ? r : r + i, ki = KC ? k + i : k;
+    const uint32_t v = (ri < rows && ki < K) ? x[ri * rs + ki * ks] : 0u;
+    w[i / 2] |= v << (16 * (i % 2));
+  }
+  out = u32x4{w[0], w[1], w[2], w[3]};
+}
+
+template <bool KC>
+__device__ __forceinline__ void lean_store(uint16_t (*dst)[kLP], int e, const u32x4& v) {
+  if constexpr (KC) {
+    *reinterpret_cast<u32x4*>(&dst[e / 8][(e % 8) * 8]) = v;
+  } else {
+    uint16_t h[8];
+    __builtin_memcpy(h, &v, 16);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) dst[(e % 8) * 8 + i][e / 8] = h[i];
+  }
+}
+
+template <bool KCA, bool KCB>
+__global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __restrict__ probs,
+                                                           int count, int64_t total_tiles,
+                                                           unsigned int* queue) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[kLT][kLP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[kLT][kLP];
+  __shared__ int64_t s_tile;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  for (;;) {
+    if (threadIdx.x == 0) s_tile = int64_t(atomicAdd(&queue[0], 1u));
+    __syncthreads();
+    const int64_t t = s_tile;
+    __syncthreads();  // everyone has read s_tile before it is rewritten
+    if (t >= total_tiles) break;
+    int lo = 0, hi = count - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (probs[mid].tile_begin <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const DevProblem& P = probs[lo];
+    int64_t tm, tn;
+    tile_coords(t - P.tile_begin, P.tiles_m, P.tiles_n, tm, tn);
+    const int64_t m0 = tm * kLT, n0 = tn * kLT;
+    const auto* A = static_cast<const uint16_t*>(P.a);
+    const auto* B = static_cast<const uint16_t*>(P.b);
+    const bool vec = P.vec != 0;
+    f32x16 acc;
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.f;
+    for (int64_t k0 = 0; k0 < P.K; k0 += kLT) {
+      u32x4 va[2], vb[2];
+#pragma unroll
+      for (int it = 0; it < 2; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        lean_load<KCA>(A, P.a_rs, P.a_ks, P.M, P.K, m0, k0, vec, e, va[it]);
+        lean_load<KCB>(B, P.b_rs, P.b_ks, P.N, P.K, n0, k0, vec, e, vb[it]);
+      }
+      if (k0 > 0) __syncthreads();  // the previous chunk is consumed
+#pragma unroll
+      for (int it = 0; it < 2; ++it) {
+        const int e = threadIdx.x + 256 * it;
+        lean_store<KCA>(As, e, va[it]);
+        lean_store<KCB>(Bs, e, vb[it]);
+      }
+      __syncthreads();
+#pragma unroll
+      for (int kk = 0; kk < kLT; kk += 16) {
+        bf16x8 a, b;
+        __builtin_memcpy(&a, &As[wm + lane % 32][kk + 8 * (lane / 32)], 16);
+        __builtin_memcpy(&b, &Bs[wn + lane % 32][kk + 8 * (lane / 32)], 16);
+        acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+      }
+    }
+    __syncthreads();  // LDS free for the next tile's first chunk
+    // Epilogue: lane holds column wn + lane % 32, rows wm + 8 (e / 4) +
+    // 4 (lane / 32) + e % 4.
+    const int64_t n = n0 + wn + lane % 32;
+    if (n < P.N) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t m = m0 + wm + 8 * q + 4 * (lane / 32);
+        float* cp = P.c + m + n * P.ldc;
+        if (P.c_vec && m + 3 < P.M) {
+          f32x4 o;
+          if (P.beta != 0.f) {
+            const f32x4 old = *reinterpret_cast<const f32x4*>(cp);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e] + P.beta * old[e];
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e];
+          }
+          *reinterpret_cast<f32x4*>(cp) = o;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (m + e < P.M) {
+              const float v = P.alpha * acc[4 * q + e];
+              cp[e] = (P.beta != 0.f) ? v + P.beta * cp[e] : v;
+            }
+        }
+      }
+    }
+  }
+  // Last workgroup out resets the queue and the exit counter.
+  if (threadIdx.x == 0) {
+    __threadfence();
+    const unsigned prev = atomicAdd(&queue[1], 1u);
+    if (prev == gridDim.x - 1) {
+      queue[0] = 0u;
+      queue[1] = 0u;
+      __threadfence();
+    }
+  }
+}
+
 using G64 = TileShape<64, 64, 64, 32, 32>;
-using G128 = TileShape<128, 128, 64, 64, 64>;
 using GF64 = TileShape<64, 64, 32, 32, 32>;
 
 struct Launch {
@@ -101,6 +243,15 @@ struct pb_grouped_plan_s {
 };
 
 namespace pbgemm {
+
+void launch_lean(pb_handle_t h, const Launch& L, unsigned grid, unsigned int* queue) {
+  switch (L.layout) {
+    case 3: hipLaunchKernelGGL((grouped_lean_kernel<true, true>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
+    case 2: hipLaunchKernelGGL((grouped_lean_kernel<true, false>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
+    case 1: hipLaunchKernelGGL((grouped_lean_kernel<false, true>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
+    default: hipLaunchKernelGGL((grouped_lean_kernel<false, false>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
+  }
+}
 
 template <typename T, typename Shape>
 void launch_grouped(pb_handle_t h, const Launch& L, unsigned grid, unsigned int* queue) {
@@ -139,7 +290,8 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
   auto* plan = new pb_grouped_plan_s();
   plan->precision = prec;
   plan->device = h->device;
-  plan->big_tiles = bf16 && 2 * big > count;
+  plan->big_tiles = false;  // BF16: the lean 64 x 64 kernel; FP32: 64 x 64 tile engine
+  (void)big;
   const int bm = plan->big_tiles ? 128 : 64, bn = bm;
   std::vector<DevProblem> by_layout[4];
   for (int64_t i = 0; i < count; ++i) {
@@ -211,12 +363,11 @@ pb_status_t pb_gemm_grouped_plan_run(pb_handle_t h, pb_grouped_plan_t plan) {
   PB_REQUIRE(plan->device == h->device, "plan belongs to another device");
   unsigned int* queue = pb::counters(h, 2);
   for (const Launch& L : plan->launches) {
-    const int64_t per_cu = plan->big_tiles ? 2 : 4;
+    const int64_t per_cu = plan->precision == PB_GEMM_BF16 ? 6 : 4;  // resident workgroups per CU
     int64_t grid = std::min<int64_t>(L.tiles, per_cu * h->num_cus);
     if (grid < 1) grid = 1;
     if (plan->precision == PB_GEMM_BF16) {
-      if (plan->big_tiles) launch_grouped<bf16_t, G128>(h, L, unsigned(grid), queue);
-      else launch_grouped<bf16_t, G64>(h, L, unsigned(grid), queue);
+      launch_lean(h, L, unsigned(grid), queue);
     } else {
       launch_grouped<float, GF64>(h, L, unsigned(grid), queue);
     }

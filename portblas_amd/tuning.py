@@ -100,20 +100,22 @@ class Autotuner:
         return i, float(t[i])
 
 
-def collect(tuner: Autotuner, shapes, progress=None):
+def collect(tuner: Autotuner, shapes, progress=None, reps: int = 3):
     """Feature matrix X (n x 8) and timing matrix T (n x configs, us)."""
     X = np.stack([features(*s) for s in shapes])
     T = np.empty((len(shapes), num_configs()))
     for i, s in enumerate(shapes):
-        T[i] = tuner.time_all(s)
+        T[i] = tuner.time_all(s, reps=reps)
         if progress and (i + 1) % progress == 0:
             print(f"[tune] {i + 1}/{len(shapes)} shapes timed", flush=True)
     return X, T
 
 
-def targets(T: np.ndarray) -> np.ndarray:
-    """Relative log runtime: log(t / min over configs)."""
-    return np.log(T / T.min(axis=1, keepdims=True)).astype(np.float32)
+def targets(T: np.ndarray, clip: float = math.log(4.0)) -> np.ndarray:
+    """Relative log runtime log(t / min over configs), clipped at `clip`:
+    configurations more than 4x slower than the best are all "bad", so the
+    network spends its capacity ranking the near-best ones."""
+    return np.minimum(np.log(T / T.min(axis=1, keepdims=True)), clip).astype(np.float32)
 
 
 @dataclass

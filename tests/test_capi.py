@@ -21,6 +21,16 @@ def test_library_built_and_loads():
     assert L.pb_abi_version() == 1
 
 
+def test_no_unresolved_internal_symbols():
+    """Every kernel launch stub and internal function the library references
+    is defined in it (an unresolved kernel stub links as a shared library but
+    fails at load or launch time)."""
+    out = subprocess.run(["nm", "-D", "-C", "--undefined-only", str(_lib.LIB_PATH)],
+                         capture_output=True, text=True, check=True).stdout
+    bad = [line for line in out.splitlines() if "pbgemm" in line or "__device_stub" in line or " pb::" in line]
+    assert not bad, bad
+
+
 def test_every_declared_symbol_is_exported():
     out = subprocess.run(["nm", "-D", "--defined-only", str(_lib.LIB_PATH)], capture_output=True,
                          text=True, check=True).stdout

@@ -270,3 +270,40 @@ def test_strided_batched_4096_x_64(handle, torch_cuda):
         ref = oracle.gemm_f64("n", "n", s, s, s, 1.0, An[sl], s, Bn[sl], s, 0.0, Cn[sl], s)
         ap = oracle.gemm_absprod_f64("n", "n", s, s, s, An[sl], s, Bn[sl], s)
         assert np.all(np.abs(Cn[sl] - ref) <= bound(s, 1.0, ap, 0.0, 0.0, U32))
+
+
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+@pytest.mark.parametrize("ta,tb", [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")])
+@pytest.mark.parametrize("m,n,k,lda_pad", [(64, 64, 64, 0), (37, 50, 200, 3), (8, 61, 5, 1)])
+def test_many_small_problems(handle, torch_cuda, dtype, ta, tb, m, n, k, lda_pad):
+    """Batches of >= one problem per CU with M, N <= 64 take the
+    one-problem-per-workgroup kernel: ragged M/N, K over several 64-chunks,
+    unaligned leading dimensions (scalar staging), alpha/beta; 300 problems,
+    checked on a sample against the FP64 oracle."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    batch = 300
+    ar, ac = (m, k) if ta == "n" else (k, m)
+    br, bc = (k, n) if tb == "n" else (n, k)
+    lda, ldb, ldc = ar + lda_pad, br + lda_pad, m + lda_pad
+    sa, sb, sc = lda * ac, ldb * bc, ldc * n
+    A = oracle.uniform(21, sa * batch).astype(dtype)
+    B = oracle.uniform(22, sb * batch).astype(dtype)
+    C = oracle.uniform(23, sc * batch).astype(dtype)
+    dC = to_dev(torch, C)
+    fn = pb._gemm_strided_batched
+    fn(handle, ta, tb, m, n, k, -1.5, to_dev(torch, A), lda, sa, to_dev(torch, B), ldb, sb, 0.25, dC, ldc, sc,
+       batch)
+    got = dC.cpu().numpy().astype(np.float64)
+    u = U32 if dtype == np.float32 else U64
+    for b in list(range(3)) + [150, batch - 1]:
+        Ab, Bb, Cb = A[b * sa:], B[b * sb:], C[b * sc: b * sc + ldc * n]
+        ref = oracle.gemm_f64(ta, tb, m, n, k, -1.5, Ab, lda, Bb, ldb, 0.25, Cb, ldc)
+        ap = colmajor(oracle.gemm_absprod_f64(ta, tb, m, n, k, Ab, lda, Bb, ldb), m, n, m)
+        tol = bound(k, -1.5, ap, 0.25, np.abs(colmajor(Cb, m, n, ldc)), u)
+        e = np.abs(colmajor(got[b * sc:], m, n, ldc) - colmajor(ref, m, n, ldc))
+        assert np.all(e <= tol), (b, float(np.max(e / tol)))
+        if ldc > m:  # padding rows untouched
+            np.testing.assert_array_equal(got[b * sc: b * sc + ldc * n].reshape(n, ldc)[:, m:],
+                                          Cb.reshape(n, ldc)[:, m:])

@@ -1,0 +1,174 @@
+"""Grouped (uneven) GEMM in one persistent launch, and hipGraph capture of
+library calls.  Grouped problems have no reference counterpart (portBLAS
+batches share one shape); each problem is checked against an FP64 product
+of its own inputs with the FP32-accumulation bound (BF16 inputs are exact in
+FP32).  Graph replays are checked against the eager result after every
+replay, with eager work and synchronisation in between."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+U = 2.0 ** -24
+
+
+class Prob(ctypes.Structure):
+    _fields_ = [("transa", ctypes.c_char), ("transb", ctypes.c_char), ("m", ctypes.c_int64),
+                ("n", ctypes.c_int64), ("k", ctypes.c_int64), ("alpha", ctypes.c_float),
+                ("beta", ctypes.c_float), ("a", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("b", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("c", ctypes.c_void_p),
+                ("ldc", ctypes.c_int64)]
+
+
+def cm(flat, rows, cols, ld):
+    return flat[: cols * ld].view(cols, ld).t()[:rows, :]
+
+
+def _make_problems(torch, dims, dtype, seed, pad=0, beta_zero=False):
+    g = torch.Generator(device="cuda").manual_seed(seed)
+    rng = np.random.default_rng(seed)
+    bufs, probs = [], (Prob * len(dims))()
+    for i, (m, n, k) in enumerate(dims):
+        ta = "n" if rng.random() < 0.5 else "t"
+        tb = "n" if rng.random() < 0.5 else "t"
+        ar, ac = (m, k) if ta == "n" else (k, m)
+        br, bc = (k, n) if tb == "n" else (n, k)
+        lda, ldb, ldc = ar + pad, br + pad, m + pad
+        a = (torch.rand(lda * ac, generator=g, device="cuda") * 2 - 1).to(dtype)
+        b = (torch.rand(ldb * bc, generator=g, device="cuda") * 2 - 1).to(dtype)
+        c = torch.rand(ldc * n, generator=g, device="cuda") * 2 - 1
+        alpha, beta = float(rng.choice([1.0, -0.5])), float(rng.choice([0.0, 0.75]))
+        beta = 0.0 if beta_zero else beta
+        probs[i] = Prob(ta.encode(), tb.encode(), m, n, k, alpha, beta, a.data_ptr(), lda, b.data_ptr(), ldb,
+                        c.data_ptr(), ldc)
+        bufs.append((ta, tb, m, n, k, alpha, beta, a, lda, b, ldb, c, c.clone(), ldc))
+    return probs, bufs
+
+
+def _check(bufs):
+    for ta, tb, m, n, k, alpha, beta, a, lda, b, ldb, c, c0, ldc in bufs:
+        ar, ac = (m, k) if ta == "n" else (k, m)
+        br, bc = (k, n) if tb == "n" else (n, k)
+        oa = cm(a.float(), ar, ac, lda).double()
+        ob = cm(b.float(), br, bc, ldb).double()
+        oa = oa if ta == "n" else oa.t()
+        ob = ob if tb == "n" else ob.t()
+        c0d = cm(c0, m, n, ldc).double()
+        ref = alpha * (oa @ ob) + beta * c0d
+        tol = 1.01 * (k + 2) * U * abs(alpha) * (oa.abs() @ ob.abs()) + 2 * U * abs(beta) * c0d.abs() + 1e-30
+        err = (cm(c, m, n, ldc).double() - ref).abs()
+        assert (err <= tol).all(), (ta, tb, m, n, k, (err / tol).max().item())
+        if ldc > m:  # padding rows untouched
+            assert ldc * n == c.numel()
+            assert (c.view(n, ldc)[:, m:] == c0.view(n, ldc)[:, m:]).all()
+
+
+@pytest.mark.parametrize("prec,dtype_name", [("bf16", "bfloat16"), ("fp32_mfma", "float32")])
+def test_grouped_uneven_problems(handle, torch_cuda, prec, dtype_name):
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    rng = np.random.default_rng(7)
+    dims = [tuple(int(v) for v in np.maximum(1, np.round(np.exp(rng.uniform(np.log(1), np.log(700), 3)))))
+            for _ in range(60)]
+    dims += [(64, 64, 64), (1, 1, 1), (513, 257, 129), (8, 1000, 3)]
+    probs, bufs = _make_problems(torch, dims, getattr(torch, dtype_name), 11, pad=3)
+    L = pb.load()
+    pb.check(L.pb_gemm_grouped(handle.handle, int(pb.gemm_precision_t[prec]), probs, len(dims)))
+    torch.cuda.synchronize()
+    _check(bufs)
+
+
+def test_grouped_plan_reuse_and_graph_replay(handle, torch_cuda):
+    """A plan runs many times; captured in a graph it replays to the eager
+    result each time (the queue resets itself, no memset node)."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    rng = np.random.default_rng(3)
+    dims = [tuple(int(v) for v in np.maximum(8, np.round(np.exp(rng.uniform(np.log(32), np.log(600), 3)) / 8) * 8))
+            for _ in range(40)]
+    probs, bufs = _make_problems(torch, dims, torch.bfloat16, 5, beta_zero=True)
+    plan = ctypes.c_void_p()
+    pb.check(L.pb_gemm_grouped_plan_create(handle.handle, int(pb.gemm_precision_t.bf16), probs, len(dims),
+                                           ctypes.byref(plan)))
+    try:
+        pb.check(L.pb_gemm_grouped_plan_run(handle.handle, plan))
+        torch.cuda.synchronize()
+        eager = [b[11].clone() for b in bufs]
+        pb.check(L.pb_gemm_grouped_plan_run(handle.handle, plan))  # second run, same result
+        torch.cuda.synchronize()
+        for e, b in zip(eager, bufs):
+            assert torch.equal(e, b[11])
+        s = torch.cuda.Stream()
+        hs = pb.SB_Handle(s)
+        graph = ctypes.c_void_p()
+        with torch.cuda.stream(s):
+            pb.check(L.pb_graph_capture_begin(hs.handle))
+            pb.check(L.pb_gemm_grouped_plan_run(hs.handle, plan))
+            pb.check(L.pb_graph_capture_end(hs.handle, ctypes.byref(graph)))
+        try:
+            for rep in range(4):
+                for b in bufs:
+                    b[11].fill_(float("nan"))
+                torch.cuda.synchronize()
+                pb.check(L.pb_graph_launch(hs.handle, graph))
+                hs.wait()
+                x = torch.ones(1 << 20, device="cuda").sum()  # eager work between replays
+                torch.cuda.synchronize()
+                assert x.item() == float(1 << 20)
+                for e, b in zip(eager, bufs):
+                    assert torch.equal(e, b[11]), f"replay {rep} differs"
+        finally:
+            L.pb_graph_destroy(graph)
+    finally:
+        L.pb_gemm_grouped_plan_destroy(plan)
+    _check(bufs)
+
+
+def test_graph_of_blas_calls(handle, torch_cuda):
+    """GEMM + GEMV + dot captured together replay to the eager results."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    s = torch.cuda.Stream()
+    hs = pb.SB_Handle(s)
+    n = 512
+    A = torch.rand(n * n, device="cuda") * 2 - 1
+    B = torch.rand(n * n, device="cuda") * 2 - 1
+    x = torch.rand(n, device="cuda") * 2 - 1
+    C = torch.zeros(n * n, device="cuda")
+    y = torch.zeros(n, device="cuda")
+    rs = torch.zeros(1, device="cuda")
+
+    def work(h):
+        pb._gemm(h, "n", "t", n, n, n, 1.0, A, n, B, n, 0.0, C, n)
+        pb._gemv(h, "n", n, n, 1.0, C, n, x, 1, 0.0, y, 1)
+        pb._dot(h, n, y, 1, y, 1, rs)
+
+    with torch.cuda.stream(s):
+        rs.zero_()
+        work(hs)
+    hs.wait()
+    C_e, y_e, rs_e = C.clone(), y.clone(), rs.clone()
+    hs.reserve_workspace(1 << 24)
+    graph = ctypes.c_void_p()
+    with torch.cuda.stream(s):
+        pb.check(L.pb_graph_capture_begin(hs.handle))
+        work(hs)
+        pb.check(L.pb_graph_capture_end(hs.handle, ctypes.byref(graph)))
+    try:
+        for rep in range(3):
+            C.zero_(); y.zero_(); rs.zero_()
+            torch.cuda.synchronize()
+            pb.check(L.pb_graph_launch(hs.handle, graph))
+            hs.wait()
+            assert torch.equal(C, C_e) and torch.equal(y, y_e) and torch.equal(rs, rs_e), rep
+    finally:
+        L.pb_graph_destroy(graph)
