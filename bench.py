@@ -50,6 +50,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--quick", action="store_true", help="headline + roofline only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--pmc-probe", action="store_true",
+                    help="for rocprofv3 --pmc: one 256 MiB device copy (the counter scale), then two "
+                         "launches of the BF16x3 kernel on packed 8192^3 operands; prints no JSON")
     return ap.parse_args()
 
 
@@ -78,6 +81,44 @@ class Timer:
             e1.synchronize()
             total += e0.elapsed_time(e1)
         return total / reps  # ms
+
+
+def committed_traffic():
+    """HBM bytes per launch of the BF16x3 kernel at 8192^3, from the
+    committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled
+    by the same counter on a copy of known size); None if absent."""
+    import re
+    path = Path(__file__).resolve().parent / "profiles" / "mt_kernel_bf16x3_8192_pmc.md"
+    if not path.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", path.read_text())
+    return float(m.group(1)) if m else None
+
+
+def pmc_probe(torch, pb, h):
+    """A plain 256 MiB copy, whose FETCH_SIZE / WRITE_SIZE give the counters'
+    scale, then the BF16x3 kernel alone on packed 8192^3 operands, twice."""
+    src = torch.ones(1 << 26, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    L = pb.load()
+    N = 8192
+    A = torch.randn(N * N, device="cuda")
+    B = torch.randn(N * N, device="cuda")
+    C = torch.zeros(N * N, device="cuda")
+    ah = torch.empty(N * N, dtype=torch.int16, device="cuda")
+    al, bh, bl = torch.empty_like(ah), torch.empty_like(ah), torch.empty_like(ah)
+    pb.check(L.pb_pack_bf16x3(h.handle, 0, b"n", N, N, A.data_ptr(), N, ah.data_ptr(), al.data_ptr(), N))
+    pb.check(L.pb_pack_bf16x3(h.handle, 1, b"n", N, N, B.data_ptr(), N, bh.data_ptr(), bl.data_ptr(), N))
+    torch.cuda.synchronize()
+    for _ in range(2):
+        pb.check(L.pb_gemm_bf16x3_packed(h.handle, N, N, N, 1.0, ah.data_ptr(), al.data_ptr(), N,
+                                         bh.data_ptr(), bl.data_ptr(), N, 0.0, C.data_ptr(), N))
+        torch.cuda.synchronize()
+    log("pmc probe done")
 
 
 def precision_study(torch, pb, h, timer, N, kind):
@@ -136,6 +177,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a GPU"
     torch.cuda.set_device(0)
     h = pb.SB_Handle()
+    if args.pmc_probe:
+        pmc_probe(torch, pb, h)
+        return
     timer = Timer(torch)
     res: dict = {}
 
@@ -179,8 +223,8 @@ def main():
     mfma_flop = 3 * flop
     achieved = mfma_flop / (t_kern * 1e-3) / 1e12
     roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["bf16"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["bf16"], "traffic": None,
-                "kernel": "mt_kernel<32,3>: 256x256 tile, BF16x3 (3 BF16 MFMA terms fused per K step)",
+                "frac": achieved / PEAK["bf16"], "traffic": committed_traffic(),
+                "kernel": "mt_kernel<16,3>: 256x256 tile, BF16x3 (3 BF16 MFMA terms fused per K step)",
                 "kernel_ms": t_kern, "flop_per_launch": mfma_flop}
     log(f"bf16 gemm kernel alone: {t_kern:.3f} ms, {achieved:.0f} TFLOP/s of BF16 MFMA "
         f"({100 * achieved / PEAK['bf16']:.1f}% of peak)")
@@ -193,6 +237,10 @@ def main():
             for name, e in d.items():
                 log(f"precision {kind:10s} {name:9s} max_rel {e['max_rel']:.3e} mean_rel {e['mean_rel']:.3e} "
                     f"normwise {e['normwise']:.3e}")
+        # max componentwise error of each split format relative to plain FP32 (<= 1: no worse)
+        res["max_rel_vs_fp32"] = {kind: {name: d[name]["max_rel"] / d["fp32_mfma"]["max_rel"]
+                                         for name in ("bf16x3", "bf16x4")}
+                                  for kind, d in res["precision_8192"].items()}
         # other variants' throughput on the headline inputs
         for name, prec in (("bf16x4", pb.gemm_precision_t.bf16x4), ("bf16", pb.gemm_precision_t.bf16),
                            ("fp8_e4m3", pb.gemm_precision_t.fp8_e4m3),
@@ -200,6 +248,8 @@ def main():
             t = timer(lambda: pb._gemm_ex(h, prec, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N), reps=3)
             res[f"sgemm_{name}_tflops"] = flop / t / 1e9
             log(f"{name}: {res[f'sgemm_{name}_tflops']:.1f} TFLOP/s")
+        res["bf16x4_speedup_vs_fp32"] = res["sgemm_bf16x4_tflops"] / res["sgemm_fp32_mfma_tflops"]
+        log(f"bf16x4 speedup vs fp32: {res['bf16x4_speedup_vs_fp32']:.2f}x")
         del A, B, C
         torch.cuda.empty_cache()
         D = 4096

@@ -142,66 +142,44 @@ __device__ __forceinline__ void pack_convert(float v, uint16_t& hi, uint16_t& lo
 }
 
 // Vectorized pre-pass for 16-byte aligned X with 4-element aligned pitches.
-// Same contract as pack_kernel.  KC (X K-contiguous): every thread converts
-// 4 consecutive k of one row straight from registers.  Otherwise (X
-// row-contiguous): 16-byte loads along the rows into an LDS tile, then every
-// thread writes 8 consecutive k of one row (8 threads cover a row's 64 k).
+// Same contract as pack_kernel.  A 64 x 64 tile of op(X) is read with
+// 16-byte loads along X's contiguous dimension (KC: along k; else along the
+// rows) into LDS as [row][k]; then thread t converts the 16 consecutive k
+// 16 (t / 64) .. +15 of row t % 64, so 64 neighbouring threads write 64
+// consecutive rows of one K slab (contiguous in the slab-major layout).
 template <int MODE, bool KC>
 __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__ x, int64_t rs,
                                                        int64_t ks, int64_t rows, int64_t K,
                                                        int64_t stride, int64_t kp, int slab,
                                                        void* dst_hi, void* dst_lo,
                                                        const float* scales, int scale_idx) {
+  __shared__ float tile[64][65];  // [row][k]
   const int64_t r0 = int64_t(blockIdx.x) * 64;
   const int64_t k0 = int64_t(blockIdx.y) * 64;
   const int64_t b = blockIdx.z;
   const float* xb = x + b * stride;
-  const float s = (MODE == kPackFp8) ? scales[scale_idx] : 1.0f;
-  auto* hi16 = static_cast<uint16_t*>(dst_hi);
-  auto* lo16 = static_cast<uint16_t*>(dst_lo);
-  auto* q8 = static_cast<uint8_t*>(dst_hi);
-  if constexpr (KC) {
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int64_t r = r0 + idx / 16, k = k0 + 4 * (idx % 16);
-      if (r >= rows || k >= kp) continue;
-      float v[4];
-      if (k + 3 < K) {
-        const pbdev::f32x4_t t =
-            pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r * rs + k));
-        v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-      } else {
+  for (int i = 0; i < 4; ++i) {
+    const int idx = threadIdx.x + 256 * i;
+    // KC: row idx / 16, k 4 (idx % 16); else k idx / 16, row 4 (idx % 16).
+    const int rr = KC ? idx / 16 : 4 * (idx % 16);
+    const int kk = KC ? 4 * (idx % 16) : idx / 16;
+    const int64_t r = r0 + rr, k = k0 + kk;
+    float v[4] = {0.f, 0.f, 0.f, 0.f};
+    if constexpr (KC) {
+      if (r < rows) {
+        if (k + 3 < K) {
+          const pbdev::f32x4_t t =
+              pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r * rs + k));
+          v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
+        } else {
 #pragma unroll
-        for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? xb[r * rs + k + e] : 0.f;
-      }
-      const int64_t o = packed_index(b, r, k, rows, kp, slab);
-      uint16_t h[4], l[4];
-      uint8_t q[4];
-#pragma unroll
-      for (int e = 0; e < 4; ++e) pack_convert<MODE>(v[e] * s, h[e], l[e], q[e]);
-      if constexpr (MODE == kPackFp8) {
-        uint32_t w;
-        __builtin_memcpy(&w, q, 4);
-        *reinterpret_cast<uint32_t*>(q8 + o) = w;
-      } else {
-        uint2 w;
-        __builtin_memcpy(&w, h, 8);
-        *reinterpret_cast<uint2*>(hi16 + o) = w;
-        if constexpr (MODE == kPackBf16Split) {
-          __builtin_memcpy(&w, l, 8);
-          *reinterpret_cast<uint2*>(lo16 + o) = w;
+          for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? xb[r * rs + k + e] : 0.f;
         }
       }
-    }
-  } else {
-    __shared__ float tile[64][65];  // [k][row]
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int idx = threadIdx.x + 256 * i;
-      const int kk = idx / 16, rq = 4 * (idx % 16);
-      const int64_t r = r0 + rq, k = k0 + kk;
-      float v[4] = {0.f, 0.f, 0.f, 0.f};
+      for (int e = 0; e < 4; ++e) tile[rr][kk + e] = v[e];
+    } else {
       if (k < K) {
         if (r + 3 < rows) {
           const pbdev::f32x4_t t =
@@ -213,33 +191,36 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
         }
       }
 #pragma unroll
-      for (int e = 0; e < 4; ++e) tile[kk][rq + e] = v[e];
+      for (int e = 0; e < 4; ++e) tile[rr + e][kk] = v[e];
     }
-    __syncthreads();
+  }
+  __syncthreads();
+  const float s = (MODE == kPackFp8) ? scales[scale_idx] : 1.0f;
+  const int rr = threadIdx.x % 64, kq = 16 * (threadIdx.x / 64);
+  const int64_t r = r0 + rr, k = k0 + kq;
+  if (r >= rows || k >= kp) return;
+  // 16 consecutive k of one row are contiguous in every packed layout (kp
+  // and the slab width are multiples of 16).
+  const int64_t o = packed_index(b, r, k, rows, kp, slab);
+  uint16_t h[16], l[16];
+  uint8_t q[16];
 #pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int item = threadIdx.x + 256 * j;
-      const int kg = item % 8, rr = item / 8;
-      const int64_t r = r0 + rr, k = k0 + 8 * kg;
-      if (r >= rows || k >= kp) continue;
-      const int64_t o = packed_index(b, r, k, rows, kp, slab);
-      uint16_t h[8], l[8];
-      uint8_t q[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) pack_convert<MODE>(tile[8 * kg + e][rr] * s, h[e], l[e], q[e]);
-      if constexpr (MODE == kPackFp8) {
-        uint2 w;
-        __builtin_memcpy(&w, q, 8);
-        *reinterpret_cast<uint2*>(q8 + o) = w;
-      } else {
-        uint4 w;
-        __builtin_memcpy(&w, h, 16);
-        *reinterpret_cast<uint4*>(hi16 + o) = w;
-        if constexpr (MODE == kPackBf16Split) {
-          __builtin_memcpy(&w, l, 16);
-          *reinterpret_cast<uint4*>(lo16 + o) = w;
-        }
-      }
+  for (int e = 0; e < 16; ++e) pack_convert<MODE>(tile[rr][kq + e] * s, h[e], l[e], q[e]);
+  if constexpr (MODE == kPackFp8) {
+    uint4 w;
+    __builtin_memcpy(&w, q, 16);
+    *reinterpret_cast<uint4*>(static_cast<uint8_t*>(dst_hi) + o) = w;
+  } else {
+    uint4 w[2];
+    __builtin_memcpy(w, h, 32);
+    uint4* dh = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_hi) + o);
+    dh[0] = w[0];
+    dh[1] = w[1];
+    if constexpr (MODE == kPackBf16Split) {
+      __builtin_memcpy(w, l, 32);
+      uint4* dl = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_lo) + o);
+      dl[0] = w[0];
+      dl[1] = w[1];
     }
   }
 }
@@ -368,9 +349,10 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   const bool fp8 = prec == PB_GEMM_FP8_E4M3;
   const bool split = prec == PB_GEMM_BF16X3 || prec == PB_GEMM_BF16X4;
   const bool use_mt = !fp8 && mt_eligible(h, M, N, K, batch);
+  // Packed pitch: the 256 x 256 kernel's ring alignment, else 16 (the
+  // vectorized pre-pass converts 16 consecutive k per thread).
   const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1),
-                                  use_mt ? (split ? mt::kKAlign<16> : mt::kKAlign<32>)
-                                         : (fp8 ? 16 : 8));
+                                  use_mt ? (split ? mt::kKAlign<16> : mt::kKAlign<32>) : 16);
   const int slab = use_mt ? (split ? 16 : 32) : 0;  // the 256 x 256 kernel's K step
   const size_t esz = fp8 ? 1 : 2;
   const int nparts = split ? 2 : 1;

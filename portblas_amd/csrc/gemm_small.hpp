@@ -2,10 +2,12 @@
 // workgroup.  The batched path of portBLAS runs its tiled kernel over the
 // batch (src/operations/blas3/gemm_local.hpp:273-279,511-516); a 64 x 64
 // problem is one tile, so what matters is latency: every thread issues all of
-// its global loads for a K chunk of 64 at once, the chunk goes through LDS
-// ([k][m] and [k][n], row pitch 80 elements so MFMA fragment reads are
-// conflict-free), and four waves each compute a 32 x 32 quarter of C with
-// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f64_16x16x4_f64 (FP64).  Small
+// its global loads for a K chunk of 64 at once, the chunk goes through LDS,
+// and four waves each compute a 32 x 32 quarter of C with
+// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f64_16x16x4_f64 (FP64).  An
+// operand keeps its contiguous dimension in LDS: row-contiguous as [k][r]
+// (pitch 80 elements, 16-byte stores), K-contiguous as [r][k] (pitch 66,
+// 8-byte stores); both make the MFMA fragment reads conflict-free.  Small
 // LDS and register footprints let 4 (FP32) or 2 (FP64) workgroups share a
 // CU.
 #pragma once
@@ -15,8 +17,16 @@
 namespace pbgemm {
 namespace small {
 
-constexpr int kTile = 64;   // M, N (and the K chunk) of one problem tile
-constexpr int kPitch = 80;  // LDS row pitch (elements)
+constexpr int kTile = 64;    // M, N (and the K chunk) of one problem tile
+constexpr int kPitchR = 80;  // [k][r] layout (row-contiguous operands)
+constexpr int kPitchK = 66;  // [r][k] layout (K-contiguous operands)
+constexpr int kLdsElems = kTile * kPitchR;
+
+// LDS index of element (r, k) of a staged operand.
+template <bool KC>
+__device__ __forceinline__ int lds_at(int r, int k) {
+  return KC ? r * kPitchK + k : k * kPitchR + r;
+}
 
 // Vector loads need 16-byte alignment of every element group they touch.
 template <typename T>
@@ -72,23 +82,33 @@ struct Chunk {
     }
   }
 
-  __device__ __forceinline__ void store(int64_t rs, T (*dst)[kPitch]) const {
+  // KC: the operand is K-contiguous (stored [r][k]); else [k][r].
+  __device__ __forceinline__ void store(int64_t rs, bool kc, T* dst) const {
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
       const int e = threadIdx.x + 256 * it;
       if (VEC && rs == 1) {
         const int r = (e % (kTile / VN)) * VN, k = e / (kTile / VN);
+        T* q = dst + lds_at<false>(r, k);
         if constexpr (sizeof(T) == 4)
-          *reinterpret_cast<pbdev::f32x4_t*>(&dst[k][r]) =
-              pbdev::f32x4_t{v[it][0], v[it][1], v[it][2], v[it][3]};
+          *reinterpret_cast<pbdev::f32x4_t*>(q) = pbdev::f32x4_t{v[it][0], v[it][1], v[it][2], v[it][3]};
         else
-          *reinterpret_cast<pbdev::f64x2_t*>(&dst[k][r]) = pbdev::f64x2_t{v[it][0], v[it][1]};
+          *reinterpret_cast<pbdev::f64x2_t*>(q) = pbdev::f64x2_t{v[it][0], v[it][1]};
       } else if (VEC) {
         const int k = (e % (kTile / VN)) * VN, r = e / (kTile / VN);
+        T* q = dst + lds_at<true>(r, k);  // 8-byte aligned (pitch 66, k a multiple of VN)
 #pragma unroll
-        for (int i = 0; i < W; ++i) dst[k + i][r] = v[it][i];
+        for (int i = 0; i < W; i += 8 / sizeof(T)) {
+          if constexpr (sizeof(T) == 4) {
+            typedef float f32x2 __attribute__((ext_vector_type(2)));
+            *reinterpret_cast<f32x2*>(q + i) = f32x2{v[it][i], v[it][i + 1]};
+          } else {
+            q[i] = v[it][i];
+          }
+        }
       } else {
-        dst[e / kTile][e % kTile] = v[it][0];
+        const int r = e % kTile, k = e / kTile;
+        dst[kc ? lds_at<true>(r, k) : lds_at<false>(r, k)] = v[it][0];
       }
     }
   }
@@ -97,8 +117,11 @@ struct Chunk {
 template <typename T, bool VEC>
 __global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
   using Acc = typename Traits<T>::AccV;  // 4 accumulators per lane (f32x4 / f64x4)
-  __shared__ __attribute__((aligned(16))) T As[kTile][kPitch];
-  __shared__ __attribute__((aligned(16))) T Bs[kTile][kPitch];
+  __shared__ __attribute__((aligned(16))) T As[kLdsElems];
+  __shared__ __attribute__((aligned(16))) T Bs[kLdsElems];
+  // Layout per operand: [r][k] when K-contiguous and vector-staged.
+  const bool akc = VEC && p.a_rs != 1 && p.a_ks == 1;
+  const bool bkc = VEC && p.b_rs != 1 && p.b_ks == 1;
   const int64_t b = blockIdx.x;
   const T* A = p.a[0] + b * p.a_stride;
   const T* B = p.b[0] + b * p.b_stride;
@@ -117,17 +140,23 @@ __global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
     ca.load(A, p.a_rs, p.a_ks, M, p.K, k0);
     cb.load(B, p.b_rs, p.b_ks, N, p.K, k0);
     if (k0 > 0) __syncthreads();  // previous chunk fully consumed
-    ca.store(p.a_rs, As);
-    cb.store(p.b_rs, Bs);
+    ca.store(p.a_rs, akc, As);
+    cb.store(p.b_rs, bkc, Bs);
     __syncthreads();
 #pragma unroll 4
     for (int k = 0; k < kTile; k += 4) {
       const int kk = k + lane / 16;
       T a[2], bv[2];
 #pragma unroll
-      for (int i = 0; i < 2; ++i) a[i] = As[kk][wm + 16 * i + lane % 16];
+      for (int i = 0; i < 2; ++i) {
+        const int r = wm + 16 * i + lane % 16;
+        a[i] = As[akc ? lds_at<true>(r, kk) : lds_at<false>(r, kk)];
+      }
 #pragma unroll
-      for (int j = 0; j < 2; ++j) bv[j] = Bs[kk][wn + 16 * j + lane % 16];
+      for (int j = 0; j < 2; ++j) {
+        const int r = wn + 16 * j + lane % 16;
+        bv[j] = Bs[bkc ? lds_at<true>(r, kk) : lds_at<false>(r, kk)];
+      }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -149,6 +178,24 @@ __global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
     for (int j = 0; j < 2; ++j) {
       const int n = wn + 16 * j + lane % 16;
       if (n >= N) continue;
+      if constexpr (sizeof(T) == 4) {
+        // FP32: accumulators e = 0..3 are 4 consecutive rows.
+        const int m = wm + 16 * i + acc_row<T>(lane, 0);
+        T* cp = C + m + int64_t(n) * p.ldc;
+        if (p.c_vec && m + 3 < M) {
+          pbdev::f32x4_t o;
+          if (beta != T(0)) {
+            const pbdev::f32x4_t old = *reinterpret_cast<const pbdev::f32x4_t*>(cp);
+            o = pbdev::f32x4_t{alpha * acc[i][j][0] + beta * old.x, alpha * acc[i][j][1] + beta * old.y,
+                               alpha * acc[i][j][2] + beta * old.z, alpha * acc[i][j][3] + beta * old.w};
+          } else {
+            o = pbdev::f32x4_t{alpha * acc[i][j][0], alpha * acc[i][j][1], alpha * acc[i][j][2],
+                               alpha * acc[i][j][3]};
+          }
+          *reinterpret_cast<pbdev::f32x4_t*>(cp) = o;
+          continue;
+        }
+      }
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int m = wm + 16 * i + acc_row<T>(lane, e);
