@@ -3,17 +3,23 @@
 // batches share one shape: src/interface/gemm_interface.hpp:202-240, with the
 // in-kernel batch loop of src/operations/blas3/gemm_local.hpp:273-279,511-516).
 //
-// The plan uploads one descriptor per problem and the prefix sum of their
-// tile counts.  A persistent grid (a few workgroups per CU) pulls tile indices
-// from a device-scope work queue, finds the owning problem by binary search
-// over the prefix sums, and runs the MFMA tile engine on that tile (the fast
-// path for interior tiles of vector-aligned problems, the guarded one
-// otherwise).  The last workgroup to leave resets the queue, so the launch
-// needs no memset and can be captured in a graph and replayed.
+// The plan uploads one descriptor per problem.
+//  BF16: the plan also balances every 64 x 64 tile over a persistent grid
+//    (six workgroups per CU) by its K, longest first, and uploads each
+//    workgroup's tile list; the lean kernel walks its list with the loads of
+//    the next chunk or tile in flight during the current one's MFMAs.  No
+//    atomics, no search, nothing to reset: the launch replays in a graph.
+//  FP32: a persistent grid pulls tile indices from a device-scope work
+//    queue, finds the owning problem by binary search over the prefix sums
+//    of the tile counts, and runs the MFMA tile engine on that tile.  The
+//    last workgroup to leave resets the queue, so the launch needs no memset
+//    and can be captured in a graph and replayed.
 #include "gemm_front.hpp"
 #include "gemm_launch.hpp"
 
 #include <algorithm>
+#include <queue>
+#include <utility>
 #include <vector>
 
 namespace pbgemm {
@@ -80,9 +86,10 @@ __global__ __launch_bounds__(256) void grouped_kernel(const DevProblem* __restri
 }
 
 // BF16 problems: a lean 64 x 64 tile.  18 KB of LDS and few registers let
-// about 8 workgroups share a CU, so the latency of one tile's loads is
-// hidden by the other resident tiles rather than by deep pipelining (most
-// grouped problems are a handful of tiles and a few K chunks).  Per K chunk
+// six or more workgroups share a CU, so the latency of one tile's loads is
+// hidden both by the other resident tiles and by issuing the next chunk's
+// loads before the current chunk's MFMAs (most grouped problems are a
+// handful of tiles and a few K chunks).  Per K chunk
 // of 64 every thread issues its 2 + 2 16-byte loads, then writes them to LDS
 // K-contiguous ([row][k], pitch 72): a K-contiguous operand with one
 // 16-byte store, an M/N-contiguous one transposed with eight 2-byte stores.
@@ -124,46 +131,61 @@ __device__ __forceinline__ void lean_store(uint16_t (*dst)[kLP], int e, const u3
   }
 }
 
+// One BF16 problem's operand sources, read once per tile so that the loads of
+// the next tile can be issued without waiting on its descriptor.
+struct LeanSrc {
+  const uint16_t* a;
+  const uint16_t* b;
+  int64_t a_rs, a_ks, b_rs, b_ks, M, N, K;
+  bool vec;
+};
+
+__device__ __forceinline__ LeanSrc lean_src(const DevProblem& P) {
+  return {static_cast<const uint16_t*>(P.a), static_cast<const uint16_t*>(P.b), P.a_rs, P.a_ks,
+          P.b_rs, P.b_ks, P.M, P.N, P.K, P.vec != 0};
+}
+
+template <bool KCA, bool KCB>
+__device__ __forceinline__ void lean_chunk_load(const LeanSrc& S, int64_t m0, int64_t n0, int64_t k0,
+                                                u32x4 (&va)[2], u32x4 (&vb)[2]) {
+#pragma unroll
+  for (int it = 0; it < 2; ++it) {
+    const int e = threadIdx.x + 256 * it;
+    lean_load<KCA>(S.a, S.a_rs, S.a_ks, S.M, S.K, m0, k0, S.vec, e, va[it]);
+    lean_load<KCB>(S.b, S.b_rs, S.b_ks, S.N, S.K, n0, k0, S.vec, e, vb[it]);
+  }
+}
+
+// Workgroup w runs tiles[starts[w] .. starts[w + 1]) (tile = {problem,
+// tile row, tile column}), a list the plan balanced by each tile's K.  The
+// loads of the next K chunk, or of the next tile's first chunk, are in flight
+// during the current chunk's MFMAs and the tile's epilogue.
 template <bool KCA, bool KCB>
 __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __restrict__ probs,
-                                                           int count, int64_t total_tiles,
-                                                           unsigned int* queue) {
+                                                           const int4* __restrict__ tiles,
+                                                           const int* __restrict__ starts) {
   __shared__ __attribute__((aligned(16))) uint16_t As[kLT][kLP];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[kLT][kLP];
-  __shared__ int64_t s_tile;
+  const int begin = starts[blockIdx.x], end = starts[blockIdx.x + 1];
+  if (begin >= end) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  for (;;) {
-    if (threadIdx.x == 0) s_tile = int64_t(atomicAdd(&queue[0], 1u));
-    __syncthreads();
-    const int64_t t = s_tile;
-    __syncthreads();  // everyone has read s_tile before it is rewritten
-    if (t >= total_tiles) break;
-    int lo = 0, hi = count - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (probs[mid].tile_begin <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const DevProblem& P = probs[lo];
-    int64_t tm, tn;
-    tile_coords(t - P.tile_begin, P.tiles_m, P.tiles_n, tm, tn);
-    const int64_t m0 = tm * kLT, n0 = tn * kLT;
-    const auto* A = static_cast<const uint16_t*>(P.a);
-    const auto* B = static_cast<const uint16_t*>(P.b);
-    const bool vec = P.vec != 0;
+  int4 cur = tiles[begin];
+  u32x4 va[2], vb[2];
+  lean_chunk_load<KCA, KCB>(lean_src(probs[cur.x]), int64_t(cur.y) * kLT, int64_t(cur.z) * kLT, 0, va, vb);
+  for (int i = begin; i < end; ++i) {
+    const DevProblem& P = probs[cur.x];
+    const LeanSrc S = lean_src(P);
+    const bool has_next = i + 1 < end;
+    const int4 nxt = has_next ? tiles[i + 1] : cur;
+    const LeanSrc Sn = lean_src(probs[nxt.x]);
+    const int64_t m0 = int64_t(cur.y) * kLT, n0 = int64_t(cur.z) * kLT;
+    const int64_t kchunks = S.K > 0 ? (S.K + kLT - 1) / kLT : 1;
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.f;
-    for (int64_t k0 = 0; k0 < P.K; k0 += kLT) {
-      u32x4 va[2], vb[2];
-#pragma unroll
-      for (int it = 0; it < 2; ++it) {
-        const int e = threadIdx.x + 256 * it;
-        lean_load<KCA>(A, P.a_rs, P.a_ks, P.M, P.K, m0, k0, vec, e, va[it]);
-        lean_load<KCB>(B, P.b_rs, P.b_ks, P.N, P.K, n0, k0, vec, e, vb[it]);
-      }
-      if (k0 > 0) __syncthreads();  // the previous chunk is consumed
+    for (int64_t kc = 0; kc < kchunks; ++kc) {
+      __syncthreads();  // the previous chunk (or tile) is consumed
 #pragma unroll
       for (int it = 0; it < 2; ++it) {
         const int e = threadIdx.x + 256 * it;
@@ -171,6 +193,10 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
         lean_store<KCB>(Bs, e, vb[it]);
       }
       __syncthreads();
+      if (kc + 1 < kchunks)
+        lean_chunk_load<KCA, KCB>(S, m0, n0, (kc + 1) * kLT, va, vb);
+      else if (has_next)
+        lean_chunk_load<KCA, KCB>(Sn, int64_t(nxt.y) * kLT, int64_t(nxt.z) * kLT, 0, va, vb);
 #pragma unroll
       for (int kk = 0; kk < kLT; kk += 16) {
         bf16x8 a, b;
@@ -179,7 +205,6 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
       }
     }
-    __syncthreads();  // LDS free for the next tile's first chunk
     // Epilogue: lane holds column wn + lane % 32, rows wm + 8 (e / 4) +
     // 4 (lane / 32) + e % 4.
     const int64_t n = n0 + wn + lane % 32;
@@ -209,16 +234,7 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
         }
       }
     }
-  }
-  // Last workgroup out resets the queue and the exit counter.
-  if (threadIdx.x == 0) {
-    __threadfence();
-    const unsigned prev = atomicAdd(&queue[1], 1u);
-    if (prev == gridDim.x - 1) {
-      queue[0] = 0u;
-      queue[1] = 0u;
-      __threadfence();
-    }
+    cur = nxt;
   }
 }
 
@@ -230,26 +246,64 @@ struct Launch {
   int count;
   int64_t tiles;
   DevProblem* dev;
+  // BF16: the balanced static schedule (grid workgroups, tile lists).
+  int grid = 0;
+  const int4* tile_list = nullptr;
+  const int* starts = nullptr;
 };
 
 }  // namespace pbgemm
 
 struct pb_grouped_plan_s {
   pb_gemm_precision_t precision;
-  bool big_tiles = false;
   std::vector<pbgemm::Launch> launches;
-  void* dev_mem = nullptr;
+  std::vector<void*> dev_mem;  // problem descriptors, tile lists, offsets
   int device = 0;
 };
 
 namespace pbgemm {
 
-void launch_lean(pb_handle_t h, const Launch& L, unsigned grid, unsigned int* queue) {
+void launch_lean(pb_handle_t h, const Launch& L) {
+  const dim3 grid(unsigned(L.grid));
   switch (L.layout) {
-    case 3: hipLaunchKernelGGL((grouped_lean_kernel<true, true>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
-    case 2: hipLaunchKernelGGL((grouped_lean_kernel<true, false>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
-    case 1: hipLaunchKernelGGL((grouped_lean_kernel<false, true>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
-    default: hipLaunchKernelGGL((grouped_lean_kernel<false, false>), dim3(grid), dim3(256), 0, h->stream, L.dev, L.count, L.tiles, queue); break;
+    case 3: hipLaunchKernelGGL((grouped_lean_kernel<true, true>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+    case 2: hipLaunchKernelGGL((grouped_lean_kernel<true, false>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+    case 1: hipLaunchKernelGGL((grouped_lean_kernel<false, true>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+    default: hipLaunchKernelGGL((grouped_lean_kernel<false, false>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+  }
+}
+
+// Balances the tiles of one BF16 launch over `grid` workgroups: longest
+// first (cost = K chunks + 1 for the epilogue), each to the least-loaded
+// workgroup.  Returns per-workgroup lists, concatenated, and their offsets.
+void balance_tiles(const std::vector<DevProblem>& v, int grid, std::vector<int4>& list,
+                   std::vector<int>& starts) {
+  struct Item { int64_t cost; int prob; };
+  std::vector<Item> probs_by_cost;
+  for (size_t i = 0; i < v.size(); ++i)
+    probs_by_cost.push_back({pb::ceil_div(std::max<int64_t>(v[i].K, 1), kLT) + 1, int(i)});
+  std::stable_sort(probs_by_cost.begin(), probs_by_cost.end(),
+                   [](const Item& x, const Item& y) { return x.cost > y.cost; });
+  using Load = std::pair<int64_t, int>;  // (assigned cost, workgroup)
+  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+  for (int w = 0; w < grid; ++w) heap.push({0, w});
+  std::vector<std::vector<int4>> per(grid);
+  for (const Item& it : probs_by_cost) {
+    const DevProblem& d = v[it.prob];
+    for (int32_t tm = 0; tm < d.tiles_m; ++tm)
+      for (int32_t tn = 0; tn < d.tiles_n; ++tn) {
+        Load l = heap.top();
+        heap.pop();
+        per[l.second].push_back(int4{it.prob, tm, tn, 0});
+        l.first += it.cost;
+        heap.push(l);
+      }
+  }
+  list.clear();
+  starts.assign(1, 0);
+  for (auto& p : per) {
+    list.insert(list.end(), p.begin(), p.end());
+    starts.push_back(int(list.size()));
   }
 }
 
@@ -282,23 +336,20 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
   PB_REQUIRE(count == 0 || problems != nullptr, "null problem array");
   const bool bf16 = prec == PB_GEMM_BF16;
   const int vw = bf16 ? 8 : 4;
-  const size_t esz = bf16 ? 2 : 4;
-  // Tile size: 128 x 128 only when most problems fill such tiles.
-  int64_t big = 0;
-  for (int64_t i = 0; i < count; ++i)
-    if (problems[i].m >= 512 && problems[i].n >= 512) ++big;
   auto* plan = new pb_grouped_plan_s();
   plan->precision = prec;
   plan->device = h->device;
-  plan->big_tiles = false;  // BF16: the lean 64 x 64 kernel; FP32: 64 x 64 tile engine
-  (void)big;
-  const int bm = plan->big_tiles ? 128 : 64, bn = bm;
+  auto destroy = [&]() {
+    for (void* m : plan->dev_mem) (void)hipFree(m);
+    delete plan;
+  };
+  constexpr int bm = 64, bn = 64;  // BF16: the lean kernel; FP32: the 64 x 64 tile engine
   std::vector<DevProblem> by_layout[4];
   for (int64_t i = 0; i < count; ++i) {
     const pb_gemm_problem_t& q = problems[i];
     const char ta = pb::norm_trans(q.transa), tb = pb::norm_trans(q.transb);
     auto fail = [&](const char* what) {
-      delete plan;
+      destroy();
       return pb::set_error(PB_STATUS_INVALID_VALUE, "problem %lld: %s", (long long)i, what);
     };
     if (!ta || !tb) return fail("invalid transpose flag");
@@ -307,6 +358,8 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     if (q.ldc < q.m || q.lda < (ta == 'n' ? q.m : q.k) || q.ldb < (tb == 'n' ? q.k : q.n))
       return fail("leading dimension too small");
     if (!q.c || (q.k > 0 && (!q.a || !q.b))) return fail("null pointer");
+    if (pb::ceil_div(q.m, bm) > 0x7fffffff || pb::ceil_div(q.n, bn) > 0x7fffffff)
+      return fail("too many tiles");
     DevProblem d{};
     d.M = q.m; d.N = q.n; d.K = q.k;
     d.alpha = q.alpha; d.beta = q.beta;
@@ -317,24 +370,29 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     d.tiles_n = int32_t(pb::ceil_div(q.n, bn));
     d.vec = aligned16(q.a) && aligned16(q.b) && q.lda % vw == 0 && q.ldb % vw == 0;
     d.c_vec = aligned16(q.c) && q.ldc % 4 == 0;
-    (void)esz;
     by_layout[2 * (ta == 't') + (tb == 'n')].push_back(d);
   }
-  size_t total = 0;
-  for (auto& v : by_layout) total += v.size();
-  if (total > 0) {
-    hipError_t e = hipMalloc(&plan->dev_mem, total * sizeof(DevProblem));
+  auto upload = [&](const void* src, size_t bytes, void** dst) -> pb_status_t {
+    hipError_t e = hipMalloc(dst, bytes);
     if (e != hipSuccess) {
-      delete plan;
+      destroy();
       return pb::set_error(PB_STATUS_ALLOC_FAILED, "plan allocation failed: %s", hipGetErrorString(e));
     }
-  }
-  DevProblem* dst = static_cast<DevProblem*>(plan->dev_mem);
+    plan->dev_mem.push_back(*dst);
+    e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      destroy();
+      return pb::set_error(PB_STATUS_DEVICE_ERROR, "plan upload failed: %s", hipGetErrorString(e));
+    }
+    return PB_STATUS_SUCCESS;
+  };
+  // Resident workgroups per CU of the BF16 lean kernel (18 KB of LDS each).
+  constexpr int64_t kLeanPerCu = 6;
   for (int lay = 3; lay >= 0; --lay) {
     auto& v = by_layout[lay];
     if (v.empty()) continue;
-    // Largest problems first: their tiles are handed out first, so the tail
-    // of the queue holds the small ones (better balance at the end).
+    // Largest problems first: for the FP32 work queue their tiles are handed
+    // out first, so the tail of the queue holds the small ones.
     std::stable_sort(v.begin(), v.end(), [](const DevProblem& x, const DevProblem& y) {
       return x.M * x.N * x.K > y.M * y.N * y.K;
     });
@@ -343,14 +401,27 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
       d.tile_begin = t;
       t += int64_t(d.tiles_m) * d.tiles_n;
     }
-    hipError_t e = hipMemcpy(dst, v.data(), v.size() * sizeof(DevProblem), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      (void)hipFree(plan->dev_mem);
-      delete plan;
-      return pb::set_error(PB_STATUS_DEVICE_ERROR, "plan upload failed: %s", hipGetErrorString(e));
+    if (bf16 && t > 0x7fffffffLL) {
+      destroy();
+      return pb::set_error(PB_STATUS_INVALID_VALUE, "grouped BF16 GEMM: more than 2^31 tiles");
     }
-    plan->launches.push_back({lay, int(v.size()), t, dst});
-    dst += v.size();
+    Launch L{lay, int(v.size()), t, nullptr};
+    void* dev = nullptr;
+    PB_TRY(upload(v.data(), v.size() * sizeof(DevProblem), &dev));
+    L.dev = static_cast<DevProblem*>(dev);
+    if (bf16) {
+      L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, kLeanPerCu * h->num_cus)));
+      std::vector<int4> list;
+      std::vector<int> starts;
+      balance_tiles(v, L.grid, list, starts);
+      void* dl = nullptr;
+      void* ds = nullptr;
+      PB_TRY(upload(list.data(), list.size() * sizeof(int4), &dl));
+      PB_TRY(upload(starts.data(), starts.size() * sizeof(int), &ds));
+      L.tile_list = static_cast<const int4*>(dl);
+      L.starts = static_cast<const int*>(ds);
+    }
+    plan->launches.push_back(L);
   }
   *out = plan;
   return PB_STATUS_SUCCESS;
@@ -363,12 +434,10 @@ pb_status_t pb_gemm_grouped_plan_run(pb_handle_t h, pb_grouped_plan_t plan) {
   PB_REQUIRE(plan->device == h->device, "plan belongs to another device");
   unsigned int* queue = pb::counters(h, 2);
   for (const Launch& L : plan->launches) {
-    const int64_t per_cu = plan->precision == PB_GEMM_BF16 ? 6 : 4;  // resident workgroups per CU
-    int64_t grid = std::min<int64_t>(L.tiles, per_cu * h->num_cus);
-    if (grid < 1) grid = 1;
     if (plan->precision == PB_GEMM_BF16) {
-      launch_lean(h, L, unsigned(grid), queue);
+      launch_lean(h, L);
     } else {
+      const int64_t grid = std::max<int64_t>(1, std::min<int64_t>(L.tiles, 4 * h->num_cus));
       launch_grouped<float, GF64>(h, L, unsigned(grid), queue);
     }
     PB_CHECK_LAUNCH();
@@ -381,7 +450,7 @@ pb_status_t pb_gemm_grouped_plan_destroy(pb_grouped_plan_t plan) {
   if (!plan) return PB_STATUS_SUCCESS;
   int cur = 0;
   if (hipGetDevice(&cur) == hipSuccess && cur != plan->device) (void)hipSetDevice(plan->device);
-  if (plan->dev_mem) (void)hipFree(plan->dev_mem);
+  for (void* m : plan->dev_mem) (void)hipFree(m);
   if (cur != plan->device) (void)hipSetDevice(cur);
   delete plan;
   return PB_STATUS_SUCCESS;
