@@ -13,9 +13,13 @@ from __future__ import annotations
 
 import ctypes
 import enum
+from pathlib import Path
 from typing import Any
 
 from . import _lib
+
+# The learned tile-configuration selector trained on MI355X timings.
+DEFAULT_SELECTOR = Path(__file__).resolve().parent / "selector_mi355x.npz"
 
 # ---------------------------------------------------------------------------
 # Enums (include/operations/blas3_trees.h:59, extension/reduction.h:40)
@@ -91,12 +95,23 @@ class SB_Handle:
     """Replaces blas::SB_Handle (include/sb_handle/portblas_handle.h:46-200).
 
     ``stream`` is a torch.cuda.Stream, a raw hipStream_t (int) or None for the
-    current torch stream (the null stream without torch)."""
+    current torch stream (the null stream without torch).
 
-    def __init__(self, stream: Any = None):
+    ``selector``: "default" installs the learned tile-configuration selector
+    trained on MI355X (``selector_mi355x.npz`` next to this module) for the
+    BF16 GEMM engine; None keeps the built-in rule; a path installs that
+    network (``portblas_amd.tuning.Selector.save`` format)."""
+
+    def __init__(self, stream: Any = None, selector: Any = "default"):
         self._L = _lib.load()
         self._h = ctypes.c_void_p()
         check(self._L.pb_handle_create(self._stream_ptr(stream), ctypes.byref(self._h)))
+        if selector is not None:
+            from . import tuning
+
+            path = DEFAULT_SELECTOR if selector == "default" else Path(selector)
+            if selector != "default" or path.exists():
+                tuning.Selector.load(path).install(self)
 
     @staticmethod
     def _stream_ptr(stream: Any) -> int:

@@ -231,19 +231,34 @@ __global__ __launch_bounds__(kBlock) void iamax_kernel(
     const V* xv = reinterpret_cast<const V*>(x);
     const int64_t step = int64_t(gridDim.x) * kBlock * kUnroll;
     int64_t base = int64_t(blockIdx.x) * kBlock * kUnroll + threadIdx.x;
-    // Full iterations: no bounds checks inside.
+    // Full iterations: no bounds checks inside.  One running best per load
+    // slot u (independent compare chains); each chain still visits its
+    // elements in increasing index order, and the chains are merged with the
+    // full better() rule below.
+    T cv[kUnroll];
+    IDX ci[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) { cv[u] = bv; ci[u] = bi; }
     for (; base + (kUnroll - 1) * kBlock < nvec; base += step) {
       V a[kUnroll];
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) a[u] = pbdev::ld_stream(xv + base + u * kBlock);
+      const IDX b0 = IDX(base * VN);
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u)
 #pragma unroll
         for (int j = 0; j < VN; ++j) {
           const T v = fabs(a[u][j]);
-          if (thread_better<IS_MAX>(v, bv)) { bv = v; bi = IDX((base + u * kBlock) * VN + j); }
+          if (thread_better<IS_MAX>(v, cv[u])) { cv[u] = v; ci[u] = b0 + IDX(u * kBlock * VN + j); }
         }
     }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u)
+      if (ci[u] != IDX(-1) &&
+          (bi == IDX(-1) || better<IS_MAX>(cv[u], int64_t(ci[u]), bv, int64_t(bi)))) {
+        bv = cv[u];
+        bi = ci[u];
+      }
     if (base < nvec) {
 #pragma unroll
       for (int u = 0; u < kUnroll; ++u) {

@@ -7,16 +7,17 @@
 // y = beta*y + alpha*sum (beta == 0 -> y not read, :185-212).
 //
 // MI355X design (HBM-bound; A is read once with non-temporal 16-byte loads):
-//  'n'  a workgroup of 8 waves owns 256 or 512 consecutive rows; every lane
-//       holds 4 consecutive rows per 256 (one 16-byte load per column for
-//       float) and the 8 waves split the workgroup's column chunk.  Partial sums are folded
-//       across waves in LDS; with several column chunks, each writes its
-//       partials to the workspace and a fold kernel sums them in a fixed
+//  'n'  a workgroup of 16 waves owns 256 consecutive rows; every lane holds
+//       4 consecutive rows (one 16-byte load per column for float) and the
+//       16 waves split the workgroup's column chunk.  Partial sums are
+//       folded across waves in LDS; with several column chunks, each writes
+//       its partials to the workspace and a fold kernel sums them in a fixed
 //       order and applies the epilogue.
 //  't'  a workgroup owns a 2048-row chunk and a group of columns; each lane
 //       keeps its 32 elements of x in registers for the whole workgroup, each
-//       wave walks its columns with 16-byte loads and reduces across the wave
-//       with shuffles.  Row chunks are folded like the 'n' column chunks.
+//       wave walks its columns, one column (8 KB, 8 16-byte loads per lane)
+//       at a time, and reduces across the wave with shuffles.  Row chunks are
+//       folded like the 'n' column chunks.
 #include "pb_internal.hpp"
 
 namespace {
@@ -56,10 +57,10 @@ __device__ __forceinline__ void store_y(T* y, int64_t idx, T alpha, T beta, T su
 // = A 16-byte aligned with lda a multiple of 4 elements; scalar: r0 + 256 q
 // + 64 e + lane).  The waves interleave the chunk's columns, kNU columns per step, so
 // each wave keeps kNU 1 KB column segments in flight.  The waves' sums are
-// folded in LDS in a fixed order.  Eight waves per workgroup keep the number
-// of column chunks (and so the fold traffic) small.
+// folded in LDS in a fixed order.  Sixteen waves per workgroup keep the
+// number of column chunks (and so the fold traffic) small.
 // ---------------------------------------------------------------------------
-constexpr int kNWaves = 8;
+constexpr int kNWaves = 16;
 constexpr int kNBlock = 64 * kNWaves;
 constexpr int kNU = 8;
 
@@ -203,28 +204,6 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
     }
   };
   int64_t j = j0 + wave;
-  if (VEC && full) {
-    // Two columns per step: 2 x kTQ 16-byte loads in flight per lane.
-    for (; j + kWaves < j1; j += 2 * kWaves) {
-      const T* ca = A + j * lda + r0 + 4 * lane;
-      const T* cb = ca + kWaves * lda;
-      Four<T> a[kTQ], b[kTQ];
-#pragma unroll
-      for (int q = 0; q < kTQ; ++q) a[q] = load4(ca + 256 * q);
-#pragma unroll
-      for (int q = 0; q < kTQ; ++q) b[q] = load4(cb + 256 * q);
-      T sa = T(0), sb = T(0);
-#pragma unroll
-      for (int q = 0; q < kTQ; ++q)
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-          sa += a[q].v[e] * xr[q][e];
-          sb += b[q].v[e] * xr[q][e];
-        }
-      emit(j, sa);
-      emit(j + kWaves, sb);
-    }
-  }
   for (; j < j1; j += kWaves) {
     const T* col = A + j * lda;
     T acc = T(0);
@@ -289,14 +268,15 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
   const bool avec = aligned16(A) && (lda * int64_t(sizeof(T))) % 16 == 0;
   const int target = 4 * h->num_cus;
   if (!tr) {
-    // About two 8-wave workgroups per CU; every chunk keeps >= 8 columns per
-    // wave.  256-row tiles (1 KB of every column per wave) measured faster
-    // than 512-row ones at 8192 x 8192.
+    // One 16-wave workgroup per CU; every chunk keeps >= 8 columns per wave.
+    // 256-row tiles (1 KB of every column per wave) measured faster than
+    // 512- or 1024-row ones at 8192 x 8192, and 16-wave workgroups faster
+    // than twice as many 8-wave ones (half the partials to fold).
     const int q = 1;
     const int64_t rows = 256 * q;
     const int64_t tiles = pb::ceil_div(M, rows);
     PB_REQUIRE(tiles <= 0x7fffffffLL, "M too large");
-    int64_t chunks = pb::ceil_div(2 * h->num_cus, tiles);
+    int64_t chunks = pb::ceil_div(h->num_cus, tiles);
     const int64_t max_chunks = pb::ceil_div(N, 8 * kNWaves);
     if (chunks > max_chunks) chunks = max_chunks;
     if (chunks < 1) chunks = 1;
