@@ -1,15 +1,16 @@
 // Many small GEMMs (M, N <= 64) of a strided batch: one problem per
 // workgroup.  The batched path of portBLAS runs its tiled kernel over the
 // batch (src/operations/blas3/gemm_local.hpp:273-279,511-516); a 64 x 64
-// problem is one tile, so what matters is latency: every thread issues all of
-// its global loads for a K chunk of 64 at once, the chunk goes through LDS,
-// and four waves each compute a 32 x 32 quarter of C with
-// v_mfma_f32_16x16x4_f32 (FP32) or v_mfma_f64_16x16x4_f64 (FP64).  An
-// operand keeps its contiguous dimension in LDS: row-contiguous as [k][r]
-// (pitch 80 elements, 16-byte stores), K-contiguous as [r][k] (pitch 66,
-// 8-byte stores); both make the MFMA fragment reads conflict-free.  Small
-// LDS and register footprints let 4 (FP32) or 2 (FP64) workgroups share a
-// CU.
+// problem is one tile, and a batch of them is bound by HBM, so what matters
+// is bytes in flight: every thread issues all of its global loads for a K
+// chunk of 32 at once, the chunk goes through LDS, the next chunk's loads are
+// issued before the current chunk's MFMAs, and four waves each compute a
+// 32 x 32 quarter of C with v_mfma_f32_16x16x4_f32 (FP32) or
+// v_mfma_f64_16x16x4_f64 (FP64).  An operand keeps its contiguous dimension
+// in LDS: row-contiguous as [k][r] (pitch 80 elements, 16-byte stores),
+// K-contiguous as [r][k] (pitch 36, 16-byte stores); with 64 LDS banks both
+// make the 16-lane x 4-k MFMA fragment reads conflict-free.  10 KB (FP32) of
+// LDS per operand lets 8 FP32 (4 FP64) workgroups share a CU.
 #pragma once
 
 #include "gemm_launch.hpp"
@@ -17,10 +18,11 @@
 namespace pbgemm {
 namespace small {
 
-constexpr int kTile = 64;    // M, N (and the K chunk) of one problem tile
+constexpr int kTile = 64;    // M, N of one problem tile
+constexpr int kKc = 32;      // K chunk staged through LDS
 constexpr int kPitchR = 80;  // [k][r] layout (row-contiguous operands)
-constexpr int kPitchK = 66;  // [r][k] layout (K-contiguous operands)
-constexpr int kLdsElems = kTile * kPitchR;
+constexpr int kPitchK = 36;  // [r][k] layout (K-contiguous operands)
+constexpr int kLdsElems = kKc * kPitchR > kTile * kPitchK ? kKc * kPitchR : kTile * kPitchK;
 
 // LDS index of element (r, k) of a staged operand.
 template <bool KC>
@@ -48,7 +50,7 @@ __device__ __forceinline__ void load_row_vec(const T* p, T out[16 / sizeof(T)]) 
 template <typename T, bool VEC>
 struct Chunk {
   static constexpr int VN = 16 / sizeof(T);
-  static constexpr int IT = VEC ? kTile * kTile / VN / 256 : kTile * kTile / 256;
+  static constexpr int IT = VEC ? kTile * kKc / VN / 256 : kTile * kKc / 256;
   static constexpr int W = VEC ? VN : 1;
   T v[IT][W];
 
@@ -67,7 +69,7 @@ struct Chunk {
             v[it][i] = (k0 + k < K && r + i < rows) ? x[(r + i) + (k0 + k) * ks] : T(0);
         }
       } else if (VEC) {  // ks == 1, vectors along k: (r, k .. k+VN-1)
-        const int k = (e % (kTile / VN)) * VN, r = e / (kTile / VN);
+        const int k = (e % (kKc / VN)) * VN, r = e / (kKc / VN);
         if (r < rows && k0 + k + VN <= K) {
           load_row_vec<T>(x + r * rs + (k0 + k), v[it]);
         } else {
@@ -95,8 +97,8 @@ struct Chunk {
         else
           *reinterpret_cast<pbdev::f64x2_t*>(q) = pbdev::f64x2_t{v[it][0], v[it][1]};
       } else if (VEC) {
-        const int k = (e % (kTile / VN)) * VN, r = e / (kTile / VN);
-        T* q = dst + lds_at<true>(r, k);  // 8-byte aligned (pitch 66, k a multiple of VN)
+        const int k = (e % (kKc / VN)) * VN, r = e / (kKc / VN);
+        T* q = dst + lds_at<true>(r, k);  // 16-byte aligned (pitch 36, k a multiple of VN)
 #pragma unroll
         for (int i = 0; i < W; i += 8 / sizeof(T)) {
           if constexpr (sizeof(T) == 4) {
@@ -135,16 +137,22 @@ __global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
-  for (int64_t k0 = 0; k0 < p.K; k0 += kTile) {
-    Chunk<T, VEC> ca, cb;
-    ca.load(A, p.a_rs, p.a_ks, M, p.K, k0);
-    cb.load(B, p.b_rs, p.b_ks, N, p.K, k0);
+  Chunk<T, VEC> ca, cb;
+  if (p.K > 0) {
+    ca.load(A, p.a_rs, p.a_ks, M, p.K, 0);
+    cb.load(B, p.b_rs, p.b_ks, N, p.K, 0);
+  }
+  for (int64_t k0 = 0; k0 < p.K; k0 += kKc) {
     if (k0 > 0) __syncthreads();  // previous chunk fully consumed
     ca.store(p.a_rs, akc, As);
     cb.store(p.b_rs, bkc, Bs);
     __syncthreads();
+    if (k0 + kKc < p.K) {  // the next chunk's loads fly during this chunk's MFMAs
+      ca.load(A, p.a_rs, p.a_ks, M, p.K, k0 + kKc);
+      cb.load(B, p.b_rs, p.b_ks, N, p.K, k0 + kKc);
+    }
 #pragma unroll 4
-    for (int k = 0; k < kTile; k += 4) {
+    for (int k = 0; k < kKc; k += 4) {
       const int kk = k + lane / 16;
       T a[2], bv[2];
 #pragma unroll
