@@ -11,7 +11,8 @@ carries the rest of the BASELINE metric and configs:
   MFMA, FP32 SIMT, BF16x3, BF16 and FP8-e4m3 against the FP64 GPU GEMM
   (pb_dgemm) on N(0,1) and log-uniform [2^-8, 2^8] inputs, and DGEMM 4096^3;
 * SGEMV 8192^2 'n'/'t' and sdot/sasum/snrm2/isamax at n = 2^26 in GB/s and as
-  a fraction of 8 TB/s (cache flushed before every timed call);
+  a fraction of 8 TB/s (cache flushed before every timed call), and the
+  streaming saxpy/scopy/sscal/sswap on the same vectors;
 * the strided batch (4096 x 64^3) and the grouped set (512 uneven BF16
   problems, one persistent launch, direct and graph-replayed);
 * `roofline` for the dominant kernel (the BF16 MFMA GEMM of the BF16x3 step,
@@ -295,6 +296,14 @@ def main():
                                  ("sasum", lambda: pb._asum(h, n, v, 1, rs[1:2]), 4 * n),
                                  ("snrm2", lambda: pb._nrm2(h, n, v, 1, rs[2:3]), 4 * n),
                                  ("isamax", lambda: pb._iamax(h, n, v, 1, ri), 4 * n)):
+            t = timer(fn, reps=10, before=flush)
+            gbs = nbytes / t / 1e6
+            bw[name] = {"GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "us": 1e3 * t}
+        # streaming BLAS-1 on the same vectors (bytes moved: axpy 3, copy 2, scal 2, swap 4 per element)
+        for name, fn, nbytes in (("saxpy", lambda: pb._axpy(h, n, 0.5, v, 1, w, 1), 12 * n),
+                                 ("scopy", lambda: pb._copy(h, n, v, 1, w, 1), 8 * n),
+                                 ("sscal", lambda: pb._scal(h, n, 1.0, w, 1), 8 * n),
+                                 ("sswap", lambda: pb._swap(h, n, v, 1, w, 1), 16 * n)):
             t = timer(fn, reps=10, before=flush)
             gbs = nbytes / t / 1e6
             bw[name] = {"GB/s": gbs, "frac_hbm": gbs / HBM_PEAK_GBS, "us": 1e3 * t}

@@ -239,6 +239,33 @@ pb_status_t pb_dasum(pb_handle_t h, int64_t n, const double* x, int64_t incx, do
 pb_status_t pb_snrm2(pb_handle_t h, int64_t n, const float* x, int64_t incx, float* rs);
 pb_status_t pb_dnrm2(pb_handle_t h, int64_t n, const double* x, int64_t incx, double* rs);
 
+/* _sdsdot: *rs <- sb + (*rs + sum x_i y_i) in FP32 (blas1_interface.h:116-125;*/
+/*       blas1_interface.hpp:148-177: the dot accumulates into *rs, then sb   */
+/*       is added); n == 0 writes sb.  The _sync form starts from zero       */
+/*       (blas1_interface.hpp:1013-1041).                                    */
+pb_status_t pb_sdsdot(pb_handle_t h, int64_t n, float sb, const float* x, int64_t incx,
+                      const float* y, int64_t incy, float* rs);
+pb_status_t pb_sdsdot_sync(pb_handle_t h, int64_t n, float sb, const float* x, int64_t incx,
+                           const float* y, int64_t incy, float* result_host);
+
+/* Streaming BLAS-1 (blas1_interface.h:45-70,193-211; blas1_interface.hpp:   */
+/* 46-74,419-456).  n <= 0 does nothing.                                     */
+/* _axpy: y <- alpha x + y   (one fused multiply-add per element)          */
+/* _scal: x <- alpha x; alpha == 0 assigns zero (NaN and inf included)      */
+/* _copy: y <- x              _swap: x <-> y                                 */
+pb_status_t pb_saxpy(pb_handle_t h, int64_t n, float alpha, const float* x, int64_t incx, float* y,
+                     int64_t incy);
+pb_status_t pb_daxpy(pb_handle_t h, int64_t n, double alpha, const double* x, int64_t incx,
+                     double* y, int64_t incy);
+pb_status_t pb_sscal(pb_handle_t h, int64_t n, float alpha, float* x, int64_t incx);
+pb_status_t pb_dscal(pb_handle_t h, int64_t n, double alpha, double* x, int64_t incx);
+pb_status_t pb_scopy(pb_handle_t h, int64_t n, const float* x, int64_t incx, float* y,
+                     int64_t incy);
+pb_status_t pb_dcopy(pb_handle_t h, int64_t n, const double* x, int64_t incx, double* y,
+                     int64_t incy);
+pb_status_t pb_sswap(pb_handle_t h, int64_t n, float* x, int64_t incx, float* y, int64_t incy);
+pb_status_t pb_dswap(pb_handle_t h, int64_t n, double* x, int64_t incx, double* y, int64_t incy);
+
 /* _iamax/_iamin (blas1_interface.h:643-667; blas1_interface.hpp:278-405;   */
 /* operators blas_operators.hpp:276-316): 0-based index of the first max   */
 /* (min) of |x_i|; the lowest index wins ties; NaN never wins; incx < 0 or  */
@@ -264,6 +291,21 @@ pb_status_t pb_isamax_sync(pb_handle_t h, int64_t n, const float* x, int64_t inc
                            int32_t* result_host);
 pb_status_t pb_isamin_sync(pb_handle_t h, int64_t n, const float* x, int64_t incx,
                            int32_t* result_host);
+
+/* ------------------------------------------------------------------------ */
+/* blas::_symm (include/interface/blas3_interface.h:78-84,138-145;          */
+/* src/interface/symm_interface.hpp:36-80): C <- alpha A B + beta C (side    */
+/* 'l', A M x M) or alpha B A + beta C (side 'r', A N x N); A symmetric,     */
+/* only its uplo ('u' / 'l') triangle is read; side and uplo are case-       */
+/* insensitive, anything else is PB_STATUS_INVALID_VALUE ("invalid _side" /  */
+/* "invalid _uplo").  alpha == 0 gives C <- beta C.                          */
+/* ------------------------------------------------------------------------ */
+pb_status_t pb_ssymm(pb_handle_t h, char side, char uplo, int64_t m, int64_t n, float alpha,
+                     const float* a, int64_t lda, const float* b, int64_t ldb, float beta, float* c,
+                     int64_t ldc);
+pb_status_t pb_dsymm(pb_handle_t h, char side, char uplo, int64_t m, int64_t n, double alpha,
+                     const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                     double* c, int64_t ldc);
 
 /* ------------------------------------------------------------------------ */
 /* blas::extension::_reduction<Op, T>                                       */

@@ -159,6 +159,17 @@ event_t _gemm_strided_batched(SB_Handle& h, char transa, char transb, int64_t M,
   });
 }
 
+template <typename T>
+event_t _symm(SB_Handle& h, char side, char uplo, int64_t M, int64_t N, T alpha, const T* a, int64_t lda,
+              const T* b, int64_t ldb, T beta, T* c, int64_t ldc, const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) {
+    if constexpr (std::is_same<T, float>::value)
+      return pb_ssymm(p, side, uplo, M, N, alpha, a, lda, b, ldb, beta, c, ldc);
+    else
+      return pb_dsymm(p, side, uplo, M, N, alpha, a, lda, b, ldb, beta, c, ldc);
+  });
+}
+
 // ---------------------------------------------------------------------------
 // BLAS-2
 // ---------------------------------------------------------------------------
@@ -183,6 +194,41 @@ event_t _dot(SB_Handle& h, int64_t N, const T* x, int64_t incx, const T* y, int6
   return detail::run(h, deps, [&](pb_handle_t p) {
     if constexpr (std::is_same<T, float>::value) return pb_sdot(p, N, x, incx, y, incy, rs);
     else return pb_ddot(p, N, x, incx, y, incy, rs);
+  });
+}
+// sdsdot: *rs <- sb + (*rs + x . y) in FP32 (blas1_interface.h:116-125).
+inline event_t _sdsdot(SB_Handle& h, int64_t N, float sb, const float* x, int64_t incx, const float* y,
+                       int64_t incy, float* rs, const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) { return pb_sdsdot(p, N, sb, x, incx, y, incy, rs); });
+}
+template <typename T>
+event_t _axpy(SB_Handle& h, int64_t N, T alpha, const T* x, int64_t incx, T* y, int64_t incy,
+              const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) {
+    if constexpr (std::is_same<T, float>::value) return pb_saxpy(p, N, alpha, x, incx, y, incy);
+    else return pb_daxpy(p, N, alpha, x, incx, y, incy);
+  });
+}
+template <typename T>
+event_t _scal(SB_Handle& h, int64_t N, T alpha, T* x, int64_t incx, const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) {
+    if constexpr (std::is_same<T, float>::value) return pb_sscal(p, N, alpha, x, incx);
+    else return pb_dscal(p, N, alpha, x, incx);
+  });
+}
+template <typename T>
+event_t _copy(SB_Handle& h, int64_t N, const T* x, int64_t incx, T* y, int64_t incy,
+              const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) {
+    if constexpr (std::is_same<T, float>::value) return pb_scopy(p, N, x, incx, y, incy);
+    else return pb_dcopy(p, N, x, incx, y, incy);
+  });
+}
+template <typename T>
+event_t _swap(SB_Handle& h, int64_t N, T* x, int64_t incx, T* y, int64_t incy, const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) {
+    if constexpr (std::is_same<T, float>::value) return pb_sswap(p, N, x, incx, y, incy);
+    else return pb_dswap(p, N, x, incx, y, incy);
   });
 }
 template <typename T>
@@ -251,6 +297,12 @@ T _dot(SB_Handle& h, int64_t N, const T* x, int64_t incx, const T* y, int64_t in
   return detail::sync_scalar<T>(h, T(0), deps, [&](T* d) {
     if constexpr (std::is_same<T, float>::value) return pb_sdot(h.native(), N, x, incx, y, incy, d);
     else return pb_ddot(h.native(), N, x, incx, y, incy, d);
+  });
+}
+inline float _sdsdot(SB_Handle& h, int64_t N, float sb, const float* x, int64_t incx, const float* y,
+                     int64_t incy, const event_t& deps = {}) {
+  return detail::sync_scalar<float>(h, 0.f, deps, [&](float* d) {
+    return pb_sdsdot(h.native(), N, sb, x, incx, y, incy, d);
   });
 }
 template <typename T>

@@ -56,6 +56,8 @@ def lib():
         L.orc_pb_sgemm.argtypes = [_ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64]
         L.orc_gemv_f64.argtypes = [_ch, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64]
         L.orc_pb_sgemv.argtypes = [_ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64]
+        L.orc_saxpy.argtypes = [_i64, _f32, _p, _i64, _p, _i64]
+        L.orc_daxpy.argtypes = [_i64, _f64, _p, _i64, _p, _i64]
         for name in ("orc_dot_f64",):
             getattr(L, name).argtypes = [_i64, _p, _i64, _p, _i64, _f64]
             getattr(L, name).restype = _f64

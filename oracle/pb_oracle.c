@@ -332,3 +332,27 @@ int orc_num_threads(void) {
   return 1;
 #endif
 }
+
+/* ------------------------------------------------------------------------ */
+/* _axpy (src/interface/blas1_interface.hpp:46-60): y <- alpha * x + y for   */
+/* each element of the views (view_sycl.hpp:95-100), evaluated with one      */
+/* rounding (a fused multiply-add, fmaf / fma), as OpenBLAS's kernels and    */
+/* contracting device compilers evaluate the expression.                    */
+/* ------------------------------------------------------------------------ */
+static int64_t orc_vidx(int64_t i, int64_t n, int64_t inc) {
+  return inc >= 0 ? i * inc : (n - 1 - i) * (-inc);
+}
+
+void orc_saxpy(int64_t n, float alpha, const float* x, int64_t incx, float* y, int64_t incy) {
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t iy = orc_vidx(i, n, incy);
+    y[iy] = fmaf(alpha, x[orc_vidx(i, n, incx)], y[iy]);
+  }
+}
+
+void orc_daxpy(int64_t n, double alpha, const double* x, int64_t incx, double* y, int64_t incy) {
+  for (int64_t i = 0; i < n; ++i) {
+    const int64_t iy = orc_vidx(i, n, incy);
+    y[iy] = fma(alpha, x[orc_vidx(i, n, incx)], y[iy]);
+  }
+}

@@ -15,6 +15,8 @@ from .blas import (  # noqa: F401
     ProductOperator,
     SB_Handle,
     _asum,
+    _axpy,
+    _copy,
     _dot,
     _gemm,
     _gemm_batched,
@@ -25,6 +27,10 @@ from .blas import (  # noqa: F401
     _iamax,
     _iamin,
     _nrm2,
+    _scal,
+    _sdsdot,
+    _swap,
+    _symm,
     check,
     extension,
     gemm_batch_type_t,

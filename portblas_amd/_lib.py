@@ -53,6 +53,8 @@ SIGNATURES: dict[str, list] = {
     "pb_gemm_grouped_plan_run": [_p, _p],
     "pb_gemm_grouped_plan_destroy": [_p],
     "pb_gemm_grouped": [_p, _i32, _p, _i64],
+    "pb_ssymm": [_p, _ch, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
+    "pb_dsymm": [_p, _ch, _ch, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64],
     "pb_sgemv": [_p, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
     "pb_dgemv": [_p, _ch, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64],
     "pb_sdot": [_p, _i64, _p, _i64, _p, _i64, _p],
@@ -61,6 +63,16 @@ SIGNATURES: dict[str, list] = {
     "pb_dasum": [_p, _i64, _p, _i64, _p],
     "pb_snrm2": [_p, _i64, _p, _i64, _p],
     "pb_dnrm2": [_p, _i64, _p, _i64, _p],
+    "pb_sdsdot": [_p, _i64, _f32, _p, _i64, _p, _i64, _p],
+    "pb_sdsdot_sync": [_p, _i64, _f32, _p, _i64, _p, _i64, ctypes.POINTER(_f32)],
+    "pb_saxpy": [_p, _i64, _f32, _p, _i64, _p, _i64],
+    "pb_daxpy": [_p, _i64, _f64, _p, _i64, _p, _i64],
+    "pb_sscal": [_p, _i64, _f32, _p, _i64],
+    "pb_dscal": [_p, _i64, _f64, _p, _i64],
+    "pb_scopy": [_p, _i64, _p, _i64, _p, _i64],
+    "pb_dcopy": [_p, _i64, _p, _i64, _p, _i64],
+    "pb_sswap": [_p, _i64, _p, _i64, _p, _i64],
+    "pb_dswap": [_p, _i64, _p, _i64, _p, _i64],
     "pb_isamax": [_p, _i64, _p, _i64, _p],
     "pb_isamin": [_p, _i64, _p, _i64, _p],
     "pb_idamax": [_p, _i64, _p, _i64, _p],

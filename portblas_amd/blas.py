@@ -200,6 +200,15 @@ def _gemm_bf16(sb_handle, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, 
                                     ldc, stridec, batch_size))
 
 
+def _symm(sb_handle, side, uplo, M, N, alpha, a, lda, b, ldb, beta, c, ldc):
+    """C <- alpha A B + beta C (side 'l') or alpha B A + beta C (side 'r'),
+    A symmetric with its `uplo` triangle stored (symm_interface.hpp:36-80)."""
+    L = sb_handle._L
+    fn = L.pb_dsymm if _kind(c) == "f64" else L.pb_ssymm
+    check(fn(sb_handle.handle, _t(side), _t(uplo), M, N, alpha, _addr(a), lda, _addr(b), ldb, beta,
+             _addr(c), ldc))
+
+
 # ---------------------------------------------------------------------------
 # BLAS-2
 # ---------------------------------------------------------------------------
@@ -252,6 +261,45 @@ def _nrm2(sb_handle, N, x, incx, rs=None):
         return out.value
     fn = L.pb_dnrm2 if _kind(x) == "f64" else L.pb_snrm2
     check(fn(sb_handle.handle, N, _addr(x), incx, _addr(rs)))
+
+
+def _sdsdot(sb_handle, N, sb, x, incx, y, incy, rs=None):
+    """rs <- sb + (rs + dot(x, y)) in FP32 (blas1_interface.hpp:148-177); the
+    synchronous form (no rs) returns sb + dot(x, y)."""
+    L = sb_handle._L
+    if rs is None:
+        out = ctypes.c_float()
+        check(L.pb_sdsdot_sync(sb_handle.handle, N, sb, _addr(x), incx, _addr(y), incy, ctypes.byref(out)))
+        return out.value
+    check(L.pb_sdsdot(sb_handle.handle, N, sb, _addr(x), incx, _addr(y), incy, _addr(rs)))
+
+
+def _axpy(sb_handle, N, alpha, x, incx, y, incy):
+    """y <- y + alpha x (blas1_interface.hpp:46-60)."""
+    L = sb_handle._L
+    fn = L.pb_daxpy if _kind(y) == "f64" else L.pb_saxpy
+    check(fn(sb_handle.handle, N, alpha, _addr(x), incx, _addr(y), incy))
+
+
+def _scal(sb_handle, N, alpha, x, incx):
+    """x <- alpha x; alpha == 0 assigns zero (blas1_interface.hpp:440-456)."""
+    L = sb_handle._L
+    fn = L.pb_dscal if _kind(x) == "f64" else L.pb_sscal
+    check(fn(sb_handle.handle, N, alpha, _addr(x), incx))
+
+
+def _copy(sb_handle, N, x, incx, y, incy):
+    """y <- x (blas1_interface.hpp:62-74)."""
+    L = sb_handle._L
+    fn = L.pb_dcopy if _kind(y) == "f64" else L.pb_scopy
+    check(fn(sb_handle.handle, N, _addr(x), incx, _addr(y), incy))
+
+
+def _swap(sb_handle, N, x, incx, y, incy):
+    """x <-> y (blas1_interface.hpp:419-430)."""
+    L = sb_handle._L
+    fn = L.pb_dswap if _kind(y) == "f64" else L.pb_sswap
+    check(fn(sb_handle.handle, N, _addr(x), incx, _addr(y), incy))
 
 
 def _iamax(sb_handle, N, x, incx, rs=None):

@@ -138,9 +138,10 @@ __global__ __launch_bounds__(kBlock) void reduce_kernel(
 // launch: stream order makes the partials visible without device-scope
 // fences, which on MI355X would write back the L2 once per workgroup) and the
 // update of *rs that portBLAS's atomics + sqrt epilogue produce.
+// add_sb: sdsdot's final *rs <- sb + *rs (blas1_interface.hpp:148-177).
 template <typename T, int OP>
 __global__ __launch_bounds__(kBlock) void reduce_fold_kernel(const T* __restrict__ partials,
-                                                             int nparts, T* rs) {
+                                                             int nparts, T* rs, T sb, int add_sb) {
   // All of a thread's loads are issued before the sums (partials come from
   // other XCDs' L2 write-back: a dependent chain would pay the latency per
   // element).
@@ -158,13 +159,24 @@ __global__ __launch_bounds__(kBlock) void reduce_fold_kernel(const T* __restrict
   }
   const T total = block_sum(s);
   if (threadIdx.x == 0) {
-    if constexpr (OP == kNrm2) rs[0] = sqrt(rs[0] + total);
-    else rs[0] = rs[0] + total;
+    if constexpr (OP == kNrm2) {
+      rs[0] = sqrt(rs[0] + total);
+    } else {
+      const T acc = rs[0] + total;
+      rs[0] = add_sb ? sb + acc : acc;
+    }
   }
 }
 
 template <typename T>
 __global__ void sqrt_inplace_kernel(T* rs) { rs[0] = sqrt(rs[0]); }
+
+// sdsdot with an empty or negative size: *rs <- sb (n == 0, the reference
+// copies sb over the result) or *rs <- sb + *rs (n < 0: the dot adds nothing).
+template <typename T>
+__global__ void sdsdot_degenerate_kernel(T* rs, T sb, int overwrite) {
+  rs[0] = overwrite ? sb : sb + rs[0];
+}
 
 // ---------------------------------------------------------------------------
 // iamax / iamin.  A candidate (v, i) beats (bv, bi) iff v is strictly better,
@@ -334,13 +346,20 @@ int grid_for(pb_handle_t h, int64_t n, int elems_per_thread_iter) {
 
 template <typename T, int OP>
 pb_status_t reduce_launch(pb_handle_t h, int64_t n, const T* x, int64_t incx,
-                          const T* y, int64_t incy, T* rs) {
+                          const T* y, int64_t incy, T* rs, const T* sb = nullptr) {
   pb::clear_error();
   PB_TRY(pb::enter(h));
   PB_REQUIRE(rs != nullptr, "null result pointer");
   if (n <= 0) {
     // dot: rs untouched (blas1_interface.hpp:621); asum adds nothing;
-    // nrm2 still takes the square root of *rs (.hpp:577-580).
+    // nrm2 still takes the square root of *rs (.hpp:577-580); sdsdot
+    // writes sb (n == 0) or adds it (.hpp:153-166).
+    if (sb) {
+      hipLaunchKernelGGL(sdsdot_degenerate_kernel<T>, dim3(1), dim3(1), 0, h->stream, rs, *sb,
+                         int(n == 0));
+      PB_CHECK_LAUNCH();
+      return PB_STATUS_SUCCESS;
+    }
     if constexpr (OP == kNrm2) {
       hipLaunchKernelGGL(sqrt_inplace_kernel<T>, dim3(1), dim3(1), 0, h->stream, rs);
       PB_CHECK_LAUNCH();
@@ -362,7 +381,7 @@ pb_status_t reduce_launch(pb_handle_t h, int64_t n, const T* x, int64_t incx,
     hipLaunchKernelGGL((reduce_kernel<T, OP, false>), dim3(grid), dim3(kBlock), 0,
                        h->stream, n, x, incx, y, incy, partials);
   hipLaunchKernelGGL((reduce_fold_kernel<T, OP>), dim3(1), dim3(kBlock), 0, h->stream, partials,
-                     grid, rs);
+                     grid, rs, sb ? *sb : T(0), int(sb != nullptr));
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }
@@ -440,6 +459,10 @@ pb_status_t pb_ddot(pb_handle_t h, int64_t n, const double* x, int64_t incx,
                     const double* y, int64_t incy, double* rs) {
   return reduce_launch<double, kDot>(h, n, x, incx, y, incy, rs);
 }
+pb_status_t pb_sdsdot(pb_handle_t h, int64_t n, float sb, const float* x, int64_t incx,
+                      const float* y, int64_t incy, float* rs) {
+  return reduce_launch<float, kDot>(h, n, x, incx, y, incy, rs, &sb);
+}
 pb_status_t pb_sasum(pb_handle_t h, int64_t n, const float* x, int64_t incx, float* rs) {
   return reduce_launch<float, kAsum>(h, n, x, incx, nullptr, 0, rs);
 }
@@ -476,6 +499,12 @@ pb_status_t pb_sdot_sync(pb_handle_t h, int64_t n, const float* x, int64_t incx,
                          const float* y, int64_t incy, float* out) {
   return sync_scalar<float>(h, 0.f, out, [&](float* d) {
     return reduce_launch<float, kDot>(h, n, x, incx, y, incy, d);
+  });
+}
+pb_status_t pb_sdsdot_sync(pb_handle_t h, int64_t n, float sb, const float* x, int64_t incx,
+                           const float* y, int64_t incy, float* out) {
+  return sync_scalar<float>(h, 0.f, out, [&](float* d) {
+    return reduce_launch<float, kDot>(h, n, x, incx, y, incy, d, &sb);
   });
 }
 pb_status_t pb_sasum_sync(pb_handle_t h, int64_t n, const float* x, int64_t incx,

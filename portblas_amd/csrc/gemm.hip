@@ -188,8 +188,26 @@ inline Plan plan_plain(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t b
   return {false, int(s)};
 }
 
+// Split-K partial bytes run_plain(h, g, false) needs, so that a caller can put
+// its own data after them in one workspace allocation (0 without a split).
 template <typename T>
-pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt) {
+size_t plain_partial_bytes(pb_handle_t h, const GemmCall<T>& g) {
+  if (small::eligible(h, g.M, g.N, g.batch, g.interleaved)) return 0;
+  if constexpr (sizeof(T) == 4) {
+    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 32);
+    return pl.big ? splitk_bytes<T, F128>(g.M, g.N, g.K, g.batch, pl.splits)
+                  : splitk_bytes<T, F64>(g.M, g.N, g.K, g.batch, pl.splits);
+  } else {
+    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 16);
+    return pl.big ? splitk_bytes<T, D128>(g.M, g.N, g.K, g.batch, pl.splits)
+                  : splitk_bytes<T, D64>(g.M, g.N, g.K, g.batch, pl.splits);
+  }
+}
+
+// `partial`: caller-provided split-K partial buffer (plain_partial_bytes), or
+// null for the handle workspace.
+template <typename T>
+pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt, T* partial = nullptr) {
   GemmProblem<T> p{};
   p.M = g.M; p.N = g.N; p.K = g.K;
   p.alpha = g.alpha; p.beta = g.beta;
@@ -198,6 +216,7 @@ pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt) {
   p.b[0] = g.b;
   p.c = g.c;
   p.batch = g.batch;
+  p.partial = partial;
   const bool vec = set_operands(p, g.ta, g.tb, g.lda, g.sa, g.ldb, g.sb, g.interleaved);
   p.c_rs = g.interleaved ? g.batch : 1;
   p.ldc = g.interleaved ? g.ldc * g.batch : g.ldc;
@@ -252,11 +271,97 @@ pb_status_t sgemm_alpha_zero(pb_handle_t h, int64_t M, int64_t N, float beta, fl
                              int64_t ldc, int64_t stridec, int64_t batch) {
   return alpha_zero<float>(h, M, N, beta, C, ldc, stridec, batch, false);
 }
+// ---------------------------------------------------------------------------
+// SYMM (src/interface/symm_interface.hpp:36-80): C <- alpha A B + beta C
+// (side 'l', A M x M) or alpha B A + beta C (side 'r', A N x N), A symmetric
+// with only its `uplo` triangle referenced.  The reference runs its GEMM with
+// a symmetric-operand loader; here the stored triangle is mirrored into a
+// full matrix in the workspace (a 64 x 64 tiled pass through LDS: k^2 reads
+// and writes, against k^2 n flops of the product) and the MFMA GEMM engine
+// runs on it.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void symmetrize_kernel(int64_t k, const T* __restrict__ a,
+                                                         int64_t lda, bool upper, T* __restrict__ full) {
+  __shared__ T mirror[64][65];  // the transposed tile (tj, ti): mirror[r][c] = A(tj*64+c, ti*64+r)
+  const int64_t i0 = int64_t(blockIdx.x) * 64, j0 = int64_t(blockIdx.y) * 64;
+  const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+  // Element (i, j) comes from A(i, j) when it is in the stored triangle
+  // (upper: i <= j), else from A(j, i).  The mirrored tile is needed when any
+  // element of the output tile lies outside the stored triangle.
+  const bool need_mirror = upper ? (i0 + 63 > j0) : (j0 + 63 > i0);
+  if (need_mirror) {
+    for (int r = ty; r < 64; r += 4) {  // A(j0 + tx, i0 + r): coalesced along tx
+      const int64_t si = j0 + tx, sj = i0 + r;
+      mirror[r][tx] = (si < k && sj < k) ? a[si + sj * lda] : T(0);
+    }
+  }
+  __syncthreads();
+  for (int c = ty; c < 64; c += 4) {
+    const int64_t i = i0 + tx, j = j0 + c;
+    if (i >= k || j >= k) continue;
+    const bool stored = upper ? (i <= j) : (i >= j);
+    full[i + j * k] = stored ? a[i + j * lda] : mirror[tx][c];
+  }
+}
+
+template <typename T>
+pb_status_t symm_entry(pb_handle_t h, char side, char uplo, int64_t M, int64_t N, T alpha,
+                       const T* a, int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  const char s = char(tolower(side)), u = char(tolower(uplo));
+  PB_REQUIRE(u == 'u' || u == 'l', "invalid _uplo");
+  PB_REQUIRE(s == 'l' || s == 'r', "invalid _side");
+  PB_REQUIRE(M >= 0 && N >= 0, "negative dimension (M=%lld N=%lld)", (long long)M, (long long)N);
+  const int64_t k = s == 'l' ? M : N;
+  PB_REQUIRE(lda >= std::max<int64_t>(1, k), "lda (%lld) < max(1, %lld)", (long long)lda, (long long)k);
+  PB_REQUIRE(ldb >= std::max<int64_t>(1, M), "ldb (%lld) < max(1, M)", (long long)ldb);
+  PB_REQUIRE(ldc >= std::max<int64_t>(1, M), "ldc (%lld) < max(1, M)", (long long)ldc);
+  if (M == 0 || N == 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(c != nullptr && a != nullptr && b != nullptr, "null matrix pointer");
+  // alpha == 0: C <- beta C (the product is not formed).
+  if (alpha == T(0)) return alpha_zero<T>(h, M, N, beta, c, ldc, ldc * N, 1, false);
+  GemmCall<T> g;
+  g.ta = 'n'; g.tb = 'n';
+  g.M = M; g.N = N; g.K = k;
+  g.alpha = alpha; g.beta = beta;
+  g.c = c; g.ldc = ldc;
+  const size_t part = pb::align_up(plain_partial_bytes<T>(h, g), 256);
+  void* ws = nullptr;
+  PB_TRY(pb::workspace(h, part + size_t(k) * size_t(k) * sizeof(T), &ws));
+  T* full = reinterpret_cast<T*>(static_cast<char*>(ws) + part);
+  const dim3 grid(unsigned(pb::ceil_div(k, 64)), unsigned(pb::ceil_div(k, 64)));
+  PB_REQUIRE(grid.y <= 65535u, "matrix too large");
+  hipLaunchKernelGGL(symmetrize_kernel<T>, grid, dim3(256), 0, h->stream, k, a, lda, u == 'u', full);
+  PB_CHECK_LAUNCH();
+  if (s == 'l') {
+    g.a = full; g.lda = k;
+    g.b = b; g.ldb = ldb;
+  } else {
+    g.a = b; g.lda = ldb;
+    g.b = full; g.ldb = k;
+  }
+  return run_plain<T>(h, g, false, part ? static_cast<T*>(ws) : nullptr);
+}
+
 }  // namespace pbgemm
 
 using namespace pbgemm;
 
 extern "C" {
+
+pb_status_t pb_ssymm(pb_handle_t h, char side, char uplo, int64_t m, int64_t n, float alpha,
+                     const float* a, int64_t lda, const float* b, int64_t ldb, float beta, float* c,
+                     int64_t ldc) {
+  return symm_entry<float>(h, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+pb_status_t pb_dsymm(pb_handle_t h, char side, char uplo, int64_t m, int64_t n, double alpha,
+                     const double* a, int64_t lda, const double* b, int64_t ldb, double beta,
+                     double* c, int64_t ldc) {
+  return symm_entry<double>(h, side, uplo, m, n, alpha, a, lda, b, ldb, beta, c, ldc);
+}
 
 pb_status_t pb_sgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
                      float alpha, const float* a, int64_t lda, const float* b, int64_t ldb,
