@@ -269,6 +269,23 @@ def main():
         t = timer(lambda: pb._gemm(h, "n", "n", M1, M1, M1, 1.5, A1, M1, B1, M1, 0.5, C1, M1), reps=20)
         res["sgemm_1024_tflops"] = 2.0 * M1 ** 3 / t / 1e9
         log(f"sgemm 1024^3 (cfg1): {res['sgemm_1024_tflops']:.1f} TFLOP/s")
+        del A1, B1, C1
+        # SYMM and TRSM on the GEMM engine (4096^2 operands)
+        S = 4096
+        As = torch.rand(S * S, device="cuda") * 2 - 1
+        Bs = torch.rand(S * S, device="cuda") * 2 - 1
+        Cs = torch.zeros(S * S, device="cuda")
+        t = timer(lambda: pb._symm(h, "l", "u", S, S, 1.0, As, S, Bs, S, 0.0, Cs, S), reps=5)
+        res["ssymm_4096_tflops"] = 2.0 * S ** 3 / t / 1e9
+        At = torch.tril(torch.rand(S, S, device="cuda") * (2.0 / S)).t().contiguous().view(-1)  # column-major lower
+        At.view(S, S).diagonal().add_(1.0)
+        X = Bs.clone()
+        t = timer(lambda: pb._trsm(h, "l", "l", "n", "n", S, S, 1.0, At, S, X, S), reps=5,
+                  before=lambda: X.copy_(Bs))
+        res["strsm_4096_tflops"] = 1.0 * S ** 3 / t / 1e9  # M^2 N flops
+        log(f"ssymm 4096^3: {res['ssymm_4096_tflops']:.1f} TFLOP/s, strsm 4096^2 x 4096: "
+            f"{res['strsm_4096_tflops']:.1f} TFLOP/s")
+        del As, Bs, Cs, At, X
 
         # ---------------- bandwidth-bound: GEMV and BLAS-1 ----------------
         # Cold caches before every timed call: read (not write) 1 GiB, so the

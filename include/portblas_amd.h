@@ -308,6 +308,22 @@ pb_status_t pb_dsymm(pb_handle_t h, char side, char uplo, int64_t m, int64_t n, 
                      double* c, int64_t ldc);
 
 /* ------------------------------------------------------------------------ */
+/* blas::_trsm (include/interface/blas3_interface.h:70-75,127-134;         */
+/* src/interface/trsm_interface.hpp:36-420): solves op(A) X = alpha B (side  */
+/* 'l', A M x M) or X op(A) = alpha B (side 'r', A N x N) and overwrites B   */
+/* with X.  A is triangular (uplo 'u' / 'l'; only that triangle is read),    */
+/* op(A) = A or A^T (trans 'n' / 't'), diag 'u' treats the diagonal as ones  */
+/* without reading it.  Flags are case-insensitive.  M, N, lda or ldb equal  */
+/* to zero, or an invalid flag, is PB_STATUS_INVALID_VALUE with the          */
+/* reference's messages ("invalid matrix size argument", "invalid Side      */
+/* argument", ...).                                                         */
+/* ------------------------------------------------------------------------ */
+pb_status_t pb_strsm(pb_handle_t h, char side, char uplo, char trans, char diag, int64_t m, int64_t n,
+                     float alpha, const float* a, int64_t lda, float* b, int64_t ldb);
+pb_status_t pb_dtrsm(pb_handle_t h, char side, char uplo, char trans, char diag, int64_t m, int64_t n,
+                     double alpha, const double* a, int64_t lda, double* b, int64_t ldb);
+
+/* ------------------------------------------------------------------------ */
 /* blas::extension::_reduction<Op, T>                                       */
 /* include/interface/extension_interface.h:453-462;                         */
 /* src/interface/extension_interface.hpp:344-410,579-594;                   */

@@ -170,6 +170,17 @@ event_t _symm(SB_Handle& h, char side, char uplo, int64_t M, int64_t N, T alpha,
   });
 }
 
+template <typename T>
+event_t _trsm(SB_Handle& h, char side, char uplo, char trans, char diag, int64_t M, int64_t N, T alpha,
+              const T* a, int64_t lda, T* b, int64_t ldb, const event_t& deps = {}) {
+  return detail::run(h, deps, [&](pb_handle_t p) {
+    if constexpr (std::is_same<T, float>::value)
+      return pb_strsm(p, side, uplo, trans, diag, M, N, alpha, a, lda, b, ldb);
+    else
+      return pb_dtrsm(p, side, uplo, trans, diag, M, N, alpha, a, lda, b, ldb);
+  });
+}
+
 // ---------------------------------------------------------------------------
 // BLAS-2
 // ---------------------------------------------------------------------------

@@ -31,6 +31,7 @@ from .blas import (  # noqa: F401
     _sdsdot,
     _swap,
     _symm,
+    _trsm,
     check,
     extension,
     gemm_batch_type_t,

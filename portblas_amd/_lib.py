@@ -55,6 +55,8 @@ SIGNATURES: dict[str, list] = {
     "pb_gemm_grouped": [_p, _i32, _p, _i64],
     "pb_ssymm": [_p, _ch, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
     "pb_dsymm": [_p, _ch, _ch, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64],
+    "pb_strsm": [_p, _ch, _ch, _ch, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64],
+    "pb_dtrsm": [_p, _ch, _ch, _ch, _ch, _i64, _i64, _f64, _p, _i64, _p, _i64],
     "pb_sgemv": [_p, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
     "pb_dgemv": [_p, _ch, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64],
     "pb_sdot": [_p, _i64, _p, _i64, _p, _i64, _p],

@@ -209,6 +209,15 @@ def _symm(sb_handle, side, uplo, M, N, alpha, a, lda, b, ldb, beta, c, ldc):
              _addr(c), ldc))
 
 
+def _trsm(sb_handle, side, uplo, trans, diag, M, N, alpha, a, lda, b, ldb):
+    """Solves op(A) X = alpha B (side 'l') or X op(A) = alpha B (side 'r'),
+    overwriting B with X (trsm_interface.hpp:36-420)."""
+    L = sb_handle._L
+    fn = L.pb_dtrsm if _kind(b) == "f64" else L.pb_strsm
+    check(fn(sb_handle.handle, _t(side), _t(uplo), _t(trans), _t(diag), M, N, alpha, _addr(a), lda,
+             _addr(b), ldb))
+
+
 # ---------------------------------------------------------------------------
 # BLAS-2
 # ---------------------------------------------------------------------------

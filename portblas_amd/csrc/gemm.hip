@@ -345,11 +345,170 @@ pb_status_t symm_entry(pb_handle_t h, char side, char uplo, int64_t M, int64_t N
   return run_plain<T>(h, g, false, part ? static_cast<T*>(ws) : nullptr);
 }
 
+// ---------------------------------------------------------------------------
+// TRSM (src/interface/trsm_interface.hpp:36-420): solves op(A) X = alpha B
+// (side 'l', A M x M) or X op(A) = alpha B (side 'r', A N x N) for X, A
+// triangular (uplo), unit or not (diag: a unit diagonal is never read),
+// and overwrites B with X.  The reference inverts 16 x 16 diagonal blocks
+// and walks them with two GEMMs per block.  Here the diagonal blocks are
+// 64 x 64 (all inverted by one launch, one workgroup per block, in LDS), so
+// a solve of order k takes ceil(k / 64) steps of two MFMA GEMMs:
+//   X_i <- op(inv(A_ii)) (alpha_i W_i)
+//   W_rest <- alpha_i W_rest - op(A)_(rest, i) X_i
+// where W is a workspace copy of B, alpha_i is alpha at the first step and 1
+// after, and the steps run forward or backward as op(A) is lower or upper.
+// ---------------------------------------------------------------------------
+constexpr int kTrsmNb = 64;
+
+// Inverts the triangular diagonal block `blk` of A (order min(64, k - 64 blk),
+// padded to 64 with the identity) into inv + blk * 64 * 64 (ld 64, same
+// triangle).  Thread c computes column c by substitution; only the stored
+// triangle (and the diagonal unless unit) of A is read.
+template <typename T>
+__global__ __launch_bounds__(64) void trsm_invert_kernel(int64_t k, const T* __restrict__ a, int64_t lda,
+                                                         bool upper, bool unit, T* __restrict__ inv) {
+  constexpr int NB = kTrsmNb;
+  // 64 KB for FP64.  Thread c reads L[i][m] (broadcast) and its own column
+  // X[m][c] (consecutive threads, consecutive banks).
+  __shared__ T L[NB][NB];  // L[i][j] = A(i0 + i, i0 + j) in the stored triangle, 0 elsewhere
+  __shared__ T X[NB][NB];  // X[i][c]: the inverse being built
+  const int64_t i0 = int64_t(blockIdx.x) * NB;
+  const int bs = int(k - i0 < NB ? k - i0 : NB);
+  const int c = threadIdx.x;
+  for (int j = 0; j < NB; ++j) {  // thread c loads row c of the block, column by column (coalesced per j)
+    const bool in_tri = upper ? (c <= j) : (c >= j);
+    T v = T(0);
+    if (c < bs && j < bs && in_tri && !(unit && c == j)) v = a[(i0 + c) + (i0 + j) * lda];
+    if (c == j && (unit || c >= bs)) v = T(1);
+    L[c][j] = v;
+  }
+  __syncthreads();
+  if (!upper) {
+    for (int i = 0; i < NB; ++i) {
+      T acc = (i == c) ? T(1) : T(0);
+      for (int m = c; m < i; ++m) acc -= L[i][m] * X[m][c];
+      X[i][c] = (i < c) ? T(0) : acc / L[i][i];
+    }
+  } else {
+    for (int i = NB - 1; i >= 0; --i) {
+      T acc = (i == c) ? T(1) : T(0);
+      for (int m = i + 1; m <= c; ++m) acc -= L[i][m] * X[m][c];
+      X[i][c] = (i > c) ? T(0) : acc / L[i][i];
+    }
+  }
+  __syncthreads();
+  T* out = inv + int64_t(blockIdx.x) * NB * NB;
+  for (int j = 0; j < NB; ++j) out[c + j * NB] = X[c][j];  // column-major, coalesced per j
+}
+
+template <typename T>
+__global__ void copy_matrix_kernel(int64_t rows, int64_t cols, const T* __restrict__ src, int64_t lds,
+                                   T* __restrict__ dst, int64_t ldd) {
+  const int64_t total = rows * cols;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = e % rows, col = e / rows;
+    dst[r + col * ldd] = src[r + col * lds];
+  }
+}
+
+template <typename T>
+pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char diag, int64_t M, int64_t N,
+                       T alpha, const T* a, int64_t lda, T* b, int64_t ldb) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  // trsm_interface.hpp:44-62
+  PB_REQUIRE(M != 0 && N != 0 && lda != 0 && ldb != 0, "invalid matrix size argument");
+  const char s = char(tolower(side)), u = char(tolower(uplo)), t = char(tolower(trans)),
+             d = char(tolower(diag));
+  PB_REQUIRE(s == 'l' || s == 'r', "invalid Side argument");
+  PB_REQUIRE(u == 'u' || u == 'l', "invalid Triangle argument");
+  PB_REQUIRE(t == 'n' || t == 't', "invalid Transpose argument");
+  PB_REQUIRE(d == 'u' || d == 'n', "invalid Diagonal argument");
+  PB_REQUIRE(M > 0 && N > 0, "negative dimension (M=%lld N=%lld)", (long long)M, (long long)N);
+  const int64_t k = s == 'l' ? M : N;
+  PB_REQUIRE(lda >= k && ldb >= M, "leading dimension too small (lda %lld, ldb %lld)", (long long)lda,
+             (long long)ldb);
+  PB_REQUIRE(a != nullptr && b != nullptr, "null matrix pointer");
+  const bool lower_eff = (u == 'l') == (t == 'n');  // op(A) is lower triangular
+  const int64_t nblk = pb::ceil_div(k, kTrsmNb);
+  PB_REQUIRE(nblk <= 0x7fffffffLL, "matrix too large");
+  // Workspace: [split-K partials of the largest GEMM step][inverted blocks][W].
+  GemmCall<T> big;
+  big.ta = t; big.tb = 'n'; big.M = M; big.N = N; big.K = kTrsmNb;
+  if (s == 'r') { big.ta = 'n'; big.tb = t; }
+  const size_t part = pb::align_up(plain_partial_bytes<T>(h, big), 256);
+  const size_t inv_bytes = pb::align_up(size_t(nblk) * kTrsmNb * kTrsmNb * sizeof(T), 256);
+  void* ws = nullptr;
+  PB_TRY(pb::workspace(h, part + inv_bytes + size_t(M) * size_t(N) * sizeof(T), &ws));
+  T* partial = part ? static_cast<T*>(ws) : nullptr;
+  T* inv = reinterpret_cast<T*>(static_cast<char*>(ws) + part);
+  T* W = reinterpret_cast<T*>(static_cast<char*>(ws) + part + inv_bytes);
+  const int64_t ldw = M;
+  hipLaunchKernelGGL(trsm_invert_kernel<T>, dim3(unsigned(nblk)), dim3(kTrsmNb), 0, h->stream, k, a, lda,
+                     u == 'u', d == 'u', inv);
+  PB_CHECK_LAUNCH();
+  {
+    int64_t g = pb::ceil_div(M * N, 256);
+    if (g > 8 * h->num_cus) g = 8 * h->num_cus;
+    hipLaunchKernelGGL(copy_matrix_kernel<T>, dim3(unsigned(g)), dim3(256), 0, h->stream, M, N, b, ldb, W, ldw);
+    PB_CHECK_LAUNCH();
+  }
+  auto gemm = [&](char ta, char tb, int64_t m, int64_t n, int64_t kk, T al, const T* A_, int64_t la,
+                  const T* B_, int64_t lb, T be, T* C_, int64_t lc) -> pb_status_t {
+    if (m <= 0 || n <= 0) return PB_STATUS_SUCCESS;
+    GemmCall<T> g;
+    g.ta = ta; g.tb = tb; g.M = m; g.N = n; g.K = kk;
+    g.alpha = al; g.beta = be;
+    g.a = A_; g.lda = la; g.b = B_; g.ldb = lb; g.c = C_; g.ldc = lc;
+    return run_plain<T>(h, g, false, partial);
+  };
+  // A(r, c) of op(A): A + r + c lda ('n') or, transposed, A + c + r lda.
+  auto opA = [&](int64_t r, int64_t c) { return t == 'n' ? a + r + c * lda : a + c + r * lda; };
+  const int64_t last = (nblk - 1) * kTrsmNb;
+  const bool forward = (s == 'l') == lower_eff;  // left+lower or right+upper walk from block 0
+  for (int64_t step = 0; step < nblk; ++step) {
+    const int64_t i = forward ? step * kTrsmNb : last - step * kTrsmNb;
+    const int64_t bs = std::min<int64_t>(kTrsmNb, k - i);
+    const T al = step == 0 ? alpha : T(1);
+    const T* invb = inv + (i / kTrsmNb) * kTrsmNb * kTrsmNb;
+    if (s == 'l') {
+      // X_i (rows i..i+bs of B) <- op(inv_ii) (al W_i)
+      PB_TRY(gemm(t, 'n', bs, N, bs, al, invb, kTrsmNb, W + i, ldw, T(0), b + i, ldb));
+      if (forward) {  // rows below
+        PB_TRY(gemm(t, 'n', M - i - bs, N, bs, T(-1), opA(i + bs, i), lda, b + i, ldb, al, W + i + bs, ldw));
+      } else {  // rows above
+        PB_TRY(gemm(t, 'n', i, N, bs, T(-1), opA(0, i), lda, b + i, ldb, al, W, ldw));
+      }
+    } else {
+      // X_i (columns i..i+bs of B) <- (al W_i) op(inv_ii)
+      PB_TRY(gemm('n', t, M, bs, bs, al, W + i * ldw, ldw, invb, kTrsmNb, T(0), b + i * ldb, ldb));
+      if (forward) {  // columns to the right
+        PB_TRY(gemm('n', t, M, N - i - bs, bs, T(-1), b + i * ldb, ldb, opA(i, i + bs), lda, al,
+                    W + (i + bs) * ldw, ldw));
+      } else {  // columns to the left
+        PB_TRY(gemm('n', t, M, i, bs, T(-1), b + i * ldb, ldb, opA(i, 0), lda, al, W, ldw));
+      }
+    }
+  }
+  return PB_STATUS_SUCCESS;
+}
+
 }  // namespace pbgemm
 
 using namespace pbgemm;
 
 extern "C" {
+
+pb_status_t pb_strsm(pb_handle_t h, char side, char uplo, char trans, char diag, int64_t m, int64_t n,
+                     float alpha, const float* a, int64_t lda, float* b, int64_t ldb) {
+  return trsm_entry<float>(h, side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
+}
+
+pb_status_t pb_dtrsm(pb_handle_t h, char side, char uplo, char trans, char diag, int64_t m, int64_t n,
+                     double alpha, const double* a, int64_t lda, double* b, int64_t ldb) {
+  return trsm_entry<double>(h, side, uplo, trans, diag, m, n, alpha, a, lda, b, ldb);
+}
 
 pb_status_t pb_ssymm(pb_handle_t h, char side, char uplo, int64_t m, int64_t n, float alpha,
                      const float* a, int64_t lda, const float* b, int64_t ldb, float beta, float* c,
