@@ -348,12 +348,13 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   const int64_t M = g.M, N = g.N, K = g.K, batch = g.batch;
   const bool fp8 = prec == PB_GEMM_FP8_E4M3;
   const bool split = prec == PB_GEMM_BF16X3 || prec == PB_GEMM_BF16X4;
-  const bool use_mt = !fp8 && mt_eligible(h, M, N, K, batch);
-  // Packed pitch: the 256 x 256 kernel's ring alignment, else 16 (the
-  // vectorized pre-pass converts 16 consecutive k per thread).
-  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1),
-                                  use_mt ? (split ? mt::kKAlign<16> : mt::kKAlign<32>) : 16);
-  const int slab = use_mt ? (split ? 16 : 32) : 0;  // the 256 x 256 kernel's K step
+  const bool use_mt = mt_eligible(h, M, N, K, batch);
+  // The 256 x 256 kernel's K step: 16 (BF16 hi/lo), 32 (BF16) or 64 (FP8)
+  // elements, 32 or 64 bytes per row.
+  const int slab = use_mt ? (split ? 16 : fp8 ? 64 : 32) : 0;
+  // Packed pitch: the 256 x 256 kernel's ring alignment (4 K steps), else 16
+  // (the vectorized pre-pass converts 16 consecutive k per thread).
+  const int64_t kp = pb::align_up(size_t(K > 0 ? K : 1), use_mt ? size_t(4 * slab) : 16);
   const size_t esz = fp8 ? 1 : 2;
   const int nparts = split ? 2 : 1;
   const size_t abytes = pb::align_up(size_t(M) * kp * esz * batch, 256);
@@ -362,7 +363,7 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   // workspace is taken: packed operands and partials share one allocation.
   int id = -1, fp8_splits = 1;
   size_t part_bytes = 0;
-  if (fp8) {
+  if (fp8 && !use_mt) {
     const int64_t t = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
     while (t * fp8_splits < 2 * h->num_cus && K / (fp8_splits * 2) >= 512 && fp8_splits < 16)
       fp8_splits *= 2;
@@ -440,6 +441,15 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   PB_CHECK_LAUNCH();
 
   const bool c_vec = aligned16(g.c) && g.ldc % 4 == 0 && (batch == 1 || g.sc % 4 == 0);
+  if (fp8 && use_mt) {
+    mt::Problem q{};
+    q.M = int(M); q.N = int(N); q.kp = int(kp);
+    q.alpha = g.alpha; q.beta = g.beta; q.alpha_dev = scales + 2;  // the dequantisation factor
+    q.a[0] = a_hi; q.b[0] = b_hi;
+    q.a_stride = M * kp; q.b_stride = N * kp;
+    q.c = g.c; q.ldc = g.ldc; q.c_stride = g.sc; q.batch = int(batch); q.c_vec = c_vec;
+    return mt::launch<64, 1, 1>(h, q);
+  }
   if (fp8) {
     GemmProblem<fp8_t> p{};
     p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;

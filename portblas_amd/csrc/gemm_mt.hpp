@@ -1,19 +1,20 @@
-// Large-tile BF16 MFMA kernel for packed operands (the hot GEMM of the
-// BF16x3 / BF16 paths at large M, N).
+// Large-tile MFMA kernel for packed operands (the hot GEMM of the BF16x3 /
+// BF16x4 / BF16 / FP8 paths at large M, N).
 //
 // Operands are the pre-pass output: op(A) (M x kp) and op(B) (N x kp) in
-// BF16, K-slab-major ([kp / BK][rows][BK], BK the kernel's K step), kp a
-// multiple of 4 K steps (64 for the split formats, 128 for BF16) with zeros
-// in [K, kp).  Up to two
-// parts per operand (hi, lo); TERMS products are accumulated per K step:
-//   TERMS 1: Ah Bh
+// BF16 or FP8 e4m3, K-slab-major ([kp / BK][rows][BK], BK the kernel's K
+// step), kp a multiple of 4 K steps (64 for the split formats, 128 for BF16,
+// 256 for FP8) with zeros in [K, kp).  Up to two parts per operand (hi, lo);
+// TERMS products are accumulated per K step:
+//   TERMS 1: Ah Bh                           (BF16, or FP8 with ESZ 1)
 //   TERMS 3: Ah Bl + Al Bh + Ah Bh           (BF16x3)
 //   TERMS 4: Ah Bl + Al Bh + Al Bl + Ah Bh   (BF16x4: adds the lo*lo term)
 //
 // Workgroup: 256 threads = 4 waves (2 x 2), C tile 256 x 256, wave tile
 // 128 x 128 = 4 x 4 v_mfma_f32_32x32x16_bf16 tiles (256 FP32 accumulators per
-// lane).  K advances in steps of BK (16 for two-part operands, 32 for BF16),
-// so one stage (every part of both operand tiles for one step) is 32 KB.
+// lane).  K advances in steps of BK (16 for two-part operands, 32 for BF16,
+// 64 for FP8 with v_mfma_scale_f32_32x32x64_f8f6f4), so one stage (every
+// part of both operand tiles for one step) is 32 KB.
 //   * global -> LDS: buffer loads straight into LDS (no VGPR staging) fill a
 //     ring of 4 stages, three steps ahead of the MFMAs (about 2 us of memory
 //     latency hidden).  The buffer range is the operand, so rows past M / N
@@ -46,8 +47,8 @@ struct Problem {
   int M, N, kp;          // kp: padded K (elements), a multiple of 4 K steps
   float alpha, beta;
   const float* alpha_dev;  // optional device factor multiplied into alpha
-  const uint16_t* a[2];    // hi, lo parts of op(A): [batch][M][kp]
-  const uint16_t* b[2];    // hi, lo parts of op(B): [batch][N][kp]
+  const void* a[2];        // hi, lo parts of op(A): [batch][kp / BK][M][BK]
+  const void* b[2];        // hi, lo parts of op(B): [batch][kp / BK][N][BK]
   int64_t a_stride, b_stride;  // batch strides (elements)
   float* c;
   int64_t ldc, c_stride;
@@ -56,20 +57,26 @@ struct Problem {
   int tiles_m, tiles_n;
 };
 
-template <int BK, int TERMS>
+// ESZ: element bytes, 2 (BF16) or 1 (FP8 e4m3).
+template <int BK, int TERMS, int ESZ = 2>
 struct Cfg {
   static constexpr int NPARTS = TERMS == 1 ? 1 : 2;
   static constexpr int PARTS = 2 * NPARTS;            // A parts, then B parts
-  static constexpr int CH = BK / 8;                   // 16-byte chunks per row
-  static constexpr int ROW_BYTES = BK * 2;
+  static constexpr int ROW_BYTES = BK * ESZ;
+  static constexpr int CH = ROW_BYTES / 16;           // 16-byte chunks per row
   static constexpr int TILE_BYTES = kBM * ROW_BYTES;  // one part of one operand
   static constexpr int STAGE_BYTES = PARTS * TILE_BYTES;
   static constexpr int NSTAGE = 4;
-  static constexpr int GROUPS = BK / 16;
+  // MFMA K groups per step: 16 (BF16, 32x32x16) or 64 (FP8, 32x32x64).
+  static constexpr int GROUPS = ESZ == 2 ? BK / 16 : BK / 64;
+  // 16-byte LDS reads per fragment (a lane's share of one MFMA operand).
+  static constexpr int FRAG_CHUNKS = ESZ == 2 ? 1 : 2;
   static constexpr int ROWS_PER_DMA = 1024 / ROW_BYTES;  // rows one wave DMA fills
   static constexpr int DMA_PER_PART = kBM / ROWS_PER_DMA;
   static constexpr int DMA_PER_WAVE = PARTS * DMA_PER_PART / 4;
-  static_assert((BK == 16 && TERMS > 1) || (BK == 32 && TERMS == 1), "BK");
+  static_assert((ESZ == 2 && ((BK == 16 && TERMS > 1) || (BK == 32 && TERMS == 1))) ||
+                    (ESZ == 1 && BK == 64 && TERMS == 1),
+                "BK");
   static_assert(STAGE_BYTES == 32 * 1024 && DMA_PER_WAVE == 8, "stage shape");
   static_assert(NSTAGE * STAGE_BYTES <= 160 * 1024, "LDS");
 };
@@ -116,16 +123,16 @@ __device__ __forceinline__ void interleave() {
   }
 }
 
-template <int TERMS> constexpr int kFragReads = TERMS == 1 ? 8 : 16;
+template <int TERMS, int ESZ> constexpr int kFragReads = (TERMS == 1 ? 8 : 16) * (ESZ == 2 ? 1 : 2);
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-template <int BK, int TERMS>
+template <int BK, int TERMS, int ESZ>
 __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   // The body is device code only (address-space casts and LDS loads have no
   // host meaning); the host pass needs just the launch stub.
 #if defined(__HIP_DEVICE_COMPILE__)
-  using C = Cfg<BK, TERMS>;
+  using C = Cfg<BK, TERMS, ESZ>;
   constexpr int NP = C::NPARTS;
   constexpr int G = C::GROUPS;
   // One array per ring stage: the compiler then tells the LDS loads of one
@@ -160,8 +167,8 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   const int rb0 = q0 % C::DMA_PER_PART;
   const bool is_a = part < NP;
   const int pidx = is_a ? part : part - NP;
-  const uint16_t* base = is_a ? p.a[pidx] + int64_t(bidx) * p.a_stride
-                              : p.b[pidx] + int64_t(bidx) * p.b_stride;
+  const uint8_t* base = is_a ? static_cast<const uint8_t*>(p.a[pidx]) + int64_t(bidx) * p.a_stride * ESZ
+                             : static_cast<const uint8_t*>(p.b[pidx]) + int64_t(bidx) * p.b_stride * ESZ;
   const uint32_t rows = uint32_t(is_a ? p.M : p.N);
   // Packed operands are K-slab-major, [kp / BK][rows][BK]: step kt of rows
   // r .. r + ROWS_PER_DMA - 1 is one contiguous kilobyte.  Each step gets its
@@ -184,7 +191,7 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   auto dma = [&](auto r, int kt) {
     const int ks = kt < nk ? kt : nk - 1;  // past the end: re-read, never consumed
     const __amdgpu_buffer_rsrc_t rsrc =
-        make_rsrc(base + int64_t(ks) * rows * BK, slab_bytes);
+        make_rsrc(base + int64_t(ks) * slab_bytes, slab_bytes);
     uint8_t* dst = stage(r) + dma_lds;
 #pragma unroll
     for (int j = 0; j < C::DMA_PER_WAVE; ++j)
@@ -197,22 +204,41 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   const int frow = lane & 31;
   const int fh = lane >> 5;
   const int fswz = swz<C::CH>(frow);
-  int foff[G];
+  // BF16: group g is chunk 2 g + fh (8 k).  FP8: the lane's 32 k of the
+  // step's 64 are chunks 2 fh and 2 fh + 1.
+  constexpr int FC = C::FRAG_CHUNKS;
+  int foff[G][FC];
 #pragma unroll
-  for (int g = 0; g < G; ++g) foff[g] = frow * C::ROW_BYTES + (((2 * g + fh) ^ fswz) * 16);
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int h = 0; h < FC; ++h) {
+      const int chunk = ESZ == 2 ? 2 * g + fh : 2 * fh + h;
+      foff[g][h] = frow * C::ROW_BYTES + ((chunk ^ fswz) * 16);
+    }
 
-  bf16x8 fa[2][NP][4], fb[2][NP][4];
+  using FragT = typename std::conditional<ESZ == 2, bf16x8, i32x8>::type;
+  FragT fa[2][NP][4], fb[2][NP][4];
+  auto ldfrag = [&](const uint8_t* b, int g, FragT& f) {
+    u32x4 v[FC];
+#pragma unroll
+    for (int h = 0; h < FC; ++h) v[h] = *reinterpret_cast<const u32x4*>(b + foff[g][h]);
+    __builtin_memcpy(&f, v, sizeof(FragT));
+  };
   auto fread = [&](auto r, int g, int buf) {
-    const uint8_t* b = stage(r) + foff[g];
+    const uint8_t* b = stage(r);
 #pragma unroll
     for (int t = 0; t < NP; ++t)
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        fa[buf][t][i] = *reinterpret_cast<const bf16x8*>(
-            b + t * C::TILE_BYTES + (wm + 32 * i) * C::ROW_BYTES);
-        fb[buf][t][i] = *reinterpret_cast<const bf16x8*>(
-            b + (NP + t) * C::TILE_BYTES + (wn + 32 * i) * C::ROW_BYTES);
+        ldfrag(b + t * C::TILE_BYTES + (wm + 32 * i) * C::ROW_BYTES, g, fa[buf][t][i]);
+        ldfrag(b + (NP + t) * C::TILE_BYTES + (wn + 32 * i) * C::ROW_BYTES, g, fb[buf][t][i]);
       }
+  };
+  // One MFMA: BF16 32x32x16, or FP8 e4m3 32x32x64 with unit scales
+  // (E8M0 127 = 2^0; the per-tensor scales are folded into alpha).
+  auto mfma = [&](const FragT& x, const FragT& y, f32x16 c) -> f32x16 {
+    if constexpr (ESZ == 2) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, c, 0, 0, 0);
+    else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x, y, c, 0, 0, 0, 127, 0, 127);
   };
 
   f32x16 acc[4][4];
@@ -231,36 +257,32 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[buf][0][i], fb[buf][1][j],
-                                                              acc[i][j], 0, 0, 0);
+          acc[i][j] = mfma(fa[buf][0][i], fb[buf][1][j], acc[i][j]);
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[buf][1][i], fb[buf][0][j],
-                                                              acc[i][j], 0, 0, 0);
+          acc[i][j] = mfma(fa[buf][1][i], fb[buf][0][j], acc[i][j]);
     }
     if constexpr (TERMS == 4) {
 #pragma unroll
       for (int i = 0; i < 4; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j)
-          acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[buf][1][i], fb[buf][1][j],
-                                                              acc[i][j], 0, 0, 0);
+          acc[i][j] = mfma(fa[buf][1][i], fb[buf][1][j], acc[i][j]);
     }
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
       for (int j = 0; j < 4; ++j)
-        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[buf][0][i], fb[buf][0][j],
-                                                            acc[i][j], 0, 0, 0);
+        acc[i][j] = mfma(fa[buf][0][i], fb[buf][0][j], acc[i][j]);
   };
 
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I3 = std::integral_constant<int, 3>;
-  constexpr int NFR = kFragReads<TERMS>;
+  constexpr int NFR = kFragReads<TERMS, ESZ>;
 
   // Prologue: steps 0..3 in flight; wait for step 0 and read its group 0.
   dma(I0{}, 0);
@@ -356,14 +378,14 @@ inline bool fits(int64_t M, int64_t N, int64_t kp) {
   return (M + kBM) * 64 < lim && (N + kBN) * 64 < lim;
 }
 
-template <int BK, int TERMS>
+template <int BK, int TERMS, int ESZ = 2>
 pb_status_t launch(pb_handle_t h, Problem p) {
   p.tiles_m = int(pb::ceil_div(p.M, kBM));
   p.tiles_n = int(pb::ceil_div(p.N, kBN));
   PB_REQUIRE(p.kp % kKAlign<BK> == 0, "packed pitch must be a multiple of %d", kKAlign<BK>);
   PB_REQUIRE(p.batch >= 1 && p.batch <= 65535, "batch out of range");
   const dim3 grid{unsigned(p.tiles_m) * unsigned(p.tiles_n), unsigned(p.batch)};
-  hipLaunchKernelGGL((mt_kernel<BK, TERMS>), grid, dim3(256), 0, h->stream, p);
+  hipLaunchKernelGGL((mt_kernel<BK, TERMS, ESZ>), grid, dim3(256), 0, h->stream, p);
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }

@@ -92,7 +92,7 @@ def test_lowp_small(handle, torch_cuda, prec, ta, tb, m, n, k):
     check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, m, n, k, 1.5, 0.5)
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "bf16x4", "bf16"])
+@pytest.mark.parametrize("prec", ["bf16x3", "bf16x4", "bf16", "fp8_e4m3"])
 @pytest.mark.parametrize("ta,tb,m,n,k,batch", [
     ("n", "n", 4096, 4096, 256, 1),   # whole 256 x 256 tiles
     ("t", "n", 4000, 4100, 1000, 1),  # ragged M, N edges; K tail inside the packed pitch
