@@ -5,7 +5,8 @@
 //
 // The plan uploads one descriptor per problem.
 //  BF16: the plan also balances every 64 x 64 tile over a persistent grid
-//    (six workgroups per CU) by its K, longest first, and uploads each
+//    (as many workgroups per CU as are resident) by its K, longest first,
+//    each problem on one XCD, and uploads each
 //    workgroup's tile list; the lean kernel walks its list with the loads of
 //    the next chunk or tile in flight during the current one's MFMAs.  No
 //    atomics, no search, nothing to reset: the launch replays in a graph.
@@ -85,8 +86,8 @@ __global__ __launch_bounds__(256) void grouped_kernel(const DevProblem* __restri
   }
 }
 
-// BF16 problems: a lean 64 x 64 tile.  18 KB of LDS and few registers let
-// six or more workgroups share a CU, so the latency of one tile's loads is
+// BF16 problems: a lean 64 x 64 tile.  18 KB of LDS and about 120 registers
+// let four workgroups share a CU, so the latency of one tile's loads is
 // hidden both by the other resident tiles and by issuing the next chunk's
 // loads before the current chunk's MFMAs (most grouped problems are a
 // handful of tiles and a few K chunks).  Per K chunk
@@ -273,36 +274,82 @@ void launch_lean(pb_handle_t h, const Launch& L) {
   }
 }
 
-// Balances the tiles of one BF16 launch over `grid` workgroups: longest
-// first (cost = K chunks + 1 for the epilogue), each to the least-loaded
-// workgroup.  Returns per-workgroup lists, concatenated, and their offsets.
+// Balances the tiles of one BF16 launch over `grid` workgroups, XCD-aware:
+// workgroup w runs on XCD w % 8, and each of the 8 XCDs has its own L2, so
+// every problem goes whole to one XCD (longest first, to the least-loaded
+// XCD) and its operand panels are fetched into one L2 only.  A problem worth
+// more than a sixteenth of the launch is spread over all XCDs instead.
+// Within an XCD the tiles go longest first to the least-loaded workgroup
+// (cost of a tile: K chunks + 1 for the epilogue).  Returns per-workgroup
+// lists, concatenated, and their offsets.
+constexpr int kXcds = 8;
+
 void balance_tiles(const std::vector<DevProblem>& v, int grid, std::vector<int4>& list,
                    std::vector<int>& starts) {
   struct Item { int64_t cost; int prob; };
   std::vector<Item> probs_by_cost;
-  for (size_t i = 0; i < v.size(); ++i)
+  int64_t total = 0;
+  for (size_t i = 0; i < v.size(); ++i) {
     probs_by_cost.push_back({pb::ceil_div(std::max<int64_t>(v[i].K, 1), kLT) + 1, int(i)});
-  std::stable_sort(probs_by_cost.begin(), probs_by_cost.end(),
-                   [](const Item& x, const Item& y) { return x.cost > y.cost; });
-  using Load = std::pair<int64_t, int>;  // (assigned cost, workgroup)
-  std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
-  for (int w = 0; w < grid; ++w) heap.push({0, w});
-  std::vector<std::vector<int4>> per(grid);
+    total += probs_by_cost.back().cost * int64_t(v[i].tiles_m) * v[i].tiles_n;
+  }
+  std::stable_sort(probs_by_cost.begin(), probs_by_cost.end(), [&](const Item& x, const Item& y) {
+    return x.cost * int64_t(v[x.prob].tiles_m) * v[x.prob].tiles_n >
+           y.cost * int64_t(v[y.prob].tiles_m) * v[y.prob].tiles_n;
+  });
+  const int xcds = grid >= kXcds ? kXcds : 1;
+  // Problems to XCDs (or "spread" for the very large ones).
+  std::vector<int64_t> xload(xcds, 0);
+  std::vector<std::vector<Item>> by_xcd(xcds);
+  std::vector<Item> spread;
   for (const Item& it : probs_by_cost) {
+    const int64_t c = it.cost * int64_t(v[it.prob].tiles_m) * v[it.prob].tiles_n;
+    if (xcds == 1 || c * 16 > total) {
+      spread.push_back(it);
+      continue;
+    }
+    const int x = int(std::min_element(xload.begin(), xload.end()) - xload.begin());
+    xload[x] += c;
+    by_xcd[x].push_back(it);
+  }
+  // Tiles to workgroups.  One heap per XCD over its workgroups (w % xcds == x).
+  using Load = std::pair<int64_t, int>;  // (assigned cost, workgroup)
+  std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> heaps(xcds);
+  for (int w = 0; w < grid; ++w) heaps[w % xcds].push({0, w});
+  std::vector<std::vector<int4>> per(grid);
+  auto place = [&](int x, const Item& it, int32_t tm, int32_t tn) {
+    Load l = heaps[x].top();
+    heaps[x].pop();
+    per[l.second].push_back(int4{it.prob, tm, tn, 0});
+    l.first += it.cost;
+    heaps[x].push(l);
+  };
+  // Spread problems first (largest), round-robin over XCDs by least load.
+  for (const Item& it : spread) {
     const DevProblem& d = v[it.prob];
     for (int32_t tm = 0; tm < d.tiles_m; ++tm)
       for (int32_t tn = 0; tn < d.tiles_n; ++tn) {
-        Load l = heap.top();
-        heap.pop();
-        per[l.second].push_back(int4{it.prob, tm, tn, 0});
-        l.first += it.cost;
-        heap.push(l);
+        int x = 0;
+        for (int y = 1; y < xcds; ++y)
+          if (heaps[y].top().first < heaps[x].top().first) x = y;
+        place(x, it, tm, tn);
       }
+  }
+  // Within an XCD: longest tiles first (a problem's tiles stay together, so
+  // the XCD's workgroups work through one problem's panels at a time).
+  for (int x = 0; x < xcds; ++x) {
+    std::stable_sort(by_xcd[x].begin(), by_xcd[x].end(),
+                     [](const Item& p1, const Item& p2) { return p1.cost > p2.cost; });
+    for (const Item& it : by_xcd[x]) {
+      const DevProblem& d = v[it.prob];
+      for (int32_t tm = 0; tm < d.tiles_m; ++tm)
+        for (int32_t tn = 0; tn < d.tiles_n; ++tn) place(x, it, tm, tn);
+    }
   }
   list.clear();
   starts.assign(1, 0);
-  for (auto& p : per) {
-    list.insert(list.end(), p.begin(), p.end());
+  for (auto& q : per) {
+    list.insert(list.end(), q.begin(), q.end());
     starts.push_back(int(list.size()));
   }
 }
@@ -386,8 +433,23 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     }
     return PB_STATUS_SUCCESS;
   };
-  // Resident workgroups per CU of the BF16 lean kernel (18 KB of LDS each).
-  constexpr int64_t kLeanPerCu = 6;
+  // The static schedule assumes every workgroup of the grid is resident at
+  // once: size the grid by the lean kernel's occupancy (registers limit it).
+  auto lean_per_cu = [&](int layout) -> int64_t {
+    int n = 0;
+    hipError_t e = hipSuccess;
+    switch (layout) {
+      case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<true, true>, 256, 0); break;
+      case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<true, false>, 256, 0); break;
+      case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<false, true>, 256, 0); break;
+      default: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<false, false>, 256, 0); break;
+    }
+    if (e != hipSuccess || n < 1) {
+      (void)hipGetLastError();
+      n = 4;
+    }
+    return n;
+  };
   for (int lay = 3; lay >= 0; --lay) {
     auto& v = by_layout[lay];
     if (v.empty()) continue;
@@ -410,7 +472,7 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     PB_TRY(upload(v.data(), v.size() * sizeof(DevProblem), &dev));
     L.dev = static_cast<DevProblem*>(dev);
     if (bf16) {
-      L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, kLeanPerCu * h->num_cus)));
+      L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, lean_per_cu(lay) * h->num_cus)));
       std::vector<int4> list;
       std::vector<int> starts;
       balance_tiles(v, L.grid, list, starts);
