@@ -1,7 +1,8 @@
 // Drop-in check of the C++ API (include/portblas_amd.hpp): the calls a
-// portBLAS user makes (blas::SB_Handle, _gemm, _gemm_strided_batched, _gemv,
-// _dot / _asum / _nrm2 / _iamax / _iamin async and sync, extension::_reduction,
-// the reference's exception types) compiled against this library, checked
+// portBLAS user makes (blas::SB_Handle, _gemm, _gemm_strided_batched, _symm,
+// _trsm, _gemv, _dot / _asum / _nrm2 / _iamax / _iamin async and sync, _axpy,
+// _scal, _copy, _swap, _sdsdot, extension::_reduction, the reference's
+// exception types) compiled against this library, checked
 // against FP64 host loops.  Built by portblas_amd/build.py; run by
 // tests/test_gpu_dropin.py.  Exit status 0 = every check passed.
 #include <hip/hip_runtime_api.h>
@@ -190,6 +191,71 @@ int main() {
     expect(inner[0] == 6 && inner[1] == 15 && inner[2] == 24, "_reduction inner");
     expect(outer[0] == 12 && outer[1] == 15 && outer[2] == 18, "_reduction outer");
     (void)hipFree(din); (void)hipFree(dout);
+  }
+
+  // Streaming BLAS-1 and sdsdot (blas1_interface.h:45-70,116-125,193-211).
+  {
+    std::vector<float> x = {1.f, 2.f, 3.f, 4.f, 5.f}, y = {10.f, 20.f, 30.f, 40.f, 50.f};
+    float* dx = upload(x);
+    float* dy = upload(y);
+    blas::_axpy(sb, 5, 2.f, dx, 1, dy, 1);
+    auto r = download(dy, 5);
+    expect(r[0] == 12.f && r[4] == 60.f, "_axpy y <- y + alpha x");
+    blas::_scal(sb, 3, -1.f, dx, 2);  // elements 0, 2, 4
+    r = download(dx, 5);
+    expect(r[0] == -1.f && r[1] == 2.f && r[2] == -3.f && r[4] == -5.f, "_scal with incx 2");
+    blas::_copy(sb, 5, dy, 1, dx, -1);  // reversed view of x
+    r = download(dx, 5);
+    expect(r[4] == 12.f && r[0] == 60.f, "_copy into a negative-increment view");
+    blas::_swap(sb, 5, dx, 1, dy, 1);
+    auto ry = download(dy, 5);
+    expect(ry[0] == 60.f && ry[4] == 12.f, "_swap");
+    const float sd = blas::_sdsdot(sb, 2, 0.5f, dx, 1, dx, 1);  // x is now {12, 24, ...}
+    expect(std::fabs(sd - (0.5f + 144.f + 576.f)) < 1e-3f, "_sdsdot sync");
+    (void)hipFree(dx); (void)hipFree(dy);
+  }
+
+  // _symm (side 'l', upper triangle stored) and _trsm (left, lower, no
+  // transpose, non-unit) against host loops; the unreferenced triangle holds NaN.
+  {
+    const int m = 40, n = 24;
+    std::vector<float> a(m * m, NAN), b = rand_vec(size_t(m) * n, 7), c(size_t(m) * n, 0.f);
+    std::vector<float> lower(m * m, NAN);
+    auto fa = rand_vec(size_t(m) * m, 8);
+    for (int j = 0; j < m; ++j)
+      for (int i = 0; i < m; ++i) {
+        if (i <= j) a[i + j * m] = fa[i + j * m];
+        if (i >= j) lower[i + j * m] = (i == j) ? 4.f : 0.1f * fa[i + j * m];
+      }
+    float *da = upload(a), *db = upload(b), *dc = upload(c), *dl = upload(lower);
+    blas::_symm(sb, 'l', 'u', m, n, 1.f, da, m, db, m, 0.f, dc, m);
+    auto got = download(dc, c.size());
+    double worst = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) {
+        double s = 0;
+        for (int k = 0; k < m; ++k) s += double(i <= k ? a[i + k * m] : a[k + i * m]) * b[k + j * m];
+        worst = std::fmax(worst, std::fabs(got[i + j * m] - s));
+      }
+    expect(worst < 1e-4, "_symm side l, upper");
+    blas::_trsm(sb, 'l', 'l', 'n', 'n', m, n, 2.f, dl, m, db, m);  // B <- X, L X = 2 B
+    auto x = download(db, b.size());
+    worst = 0;
+    for (int j = 0; j < n; ++j)
+      for (int i = 0; i < m; ++i) {
+        double s = 0;
+        for (int k = 0; k <= i; ++k) s += double(lower[i + k * m]) * x[k + j * m];
+        worst = std::fmax(worst, std::fabs(s - 2.0 * b[i + j * m]));
+      }
+    expect(worst < 1e-4, "_trsm left, lower: residual");
+    bool threw = false;
+    try {
+      blas::_trsm(sb, 'x', 'l', 'n', 'n', m, n, 1.f, dl, m, db, m);
+    } catch (const std::invalid_argument&) {
+      threw = true;
+    }
+    expect(threw, "_trsm bad side throws std::invalid_argument");
+    (void)hipFree(da); (void)hipFree(db); (void)hipFree(dc); (void)hipFree(dl);
   }
 
   // Exceptions: the reference throws std::invalid_argument on a bad trans.
