@@ -293,6 +293,22 @@ pb_status_t pb_isamin_sync(pb_handle_t h, int64_t n, const float* x, int64_t inc
                            int32_t* result_host);
 
 /* ------------------------------------------------------------------------ */
+/* Complex GEMM: blas::_gemm on std::complex<float / double> (built with     */
+/* BLAS_ENABLE_COMPLEX; src/interface/gemm_interface.hpp:58-64,140-198,     */
+/* test/unittest/blas3/blas3_gemm_common.hpp:480-580).  Matrices hold        */
+/* interleaved (re, im) pairs, leading dimensions count complex elements,     */
+/* alpha and beta point to (re, im) pairs in host memory.  'c' is taken as   */
+/* 't' without conjugation, as the reference does; beta == 0 never reads C; */
+/* alpha == 0 (or K == 0) gives C <- beta C.                                */
+/* ------------------------------------------------------------------------ */
+pb_status_t pb_cgemm(pb_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                     const float* alpha, const float* a, int64_t lda, const float* b, int64_t ldb,
+                     const float* beta, float* c, int64_t ldc);
+pb_status_t pb_zgemm(pb_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                     const double* alpha, const double* a, int64_t lda, const double* b, int64_t ldb,
+                     const double* beta, double* c, int64_t ldc);
+
+/* ------------------------------------------------------------------------ */
 /* blas::_symm (include/interface/blas3_interface.h:78-84,138-145;          */
 /* src/interface/symm_interface.hpp:36-80): C <- alpha A B + beta C (side    */
 /* 'l', A M x M) or alpha B A + beta C (side 'r', A N x N); A symmetric,     */

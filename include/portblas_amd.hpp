@@ -19,6 +19,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <complex>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -117,10 +118,20 @@ template <typename T>
 event_t _gemm(SB_Handle& h, char transa, char transb, int64_t M, int64_t N, int64_t K, T alpha,
               const T* a, int64_t lda, const T* b, int64_t ldb, T beta, T* c, int64_t ldc,
               const event_t& deps = {}) {
-  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value,
-                "float or double");
+  static_assert(std::is_same<T, float>::value || std::is_same<T, double>::value ||
+                    std::is_same<T, std::complex<float>>::value ||
+                    std::is_same<T, std::complex<double>>::value,
+                "float, double, std::complex<float> or std::complex<double>");
   return detail::run(h, deps, [&](pb_handle_t p) {
-    if constexpr (std::is_same<T, float>::value)
+    if constexpr (std::is_same<T, std::complex<float>>::value)
+      return pb_cgemm(p, transa, transb, M, N, K, reinterpret_cast<const float*>(&alpha),
+                      reinterpret_cast<const float*>(a), lda, reinterpret_cast<const float*>(b), ldb,
+                      reinterpret_cast<const float*>(&beta), reinterpret_cast<float*>(c), ldc);
+    else if constexpr (std::is_same<T, std::complex<double>>::value)
+      return pb_zgemm(p, transa, transb, M, N, K, reinterpret_cast<const double*>(&alpha),
+                      reinterpret_cast<const double*>(a), lda, reinterpret_cast<const double*>(b), ldb,
+                      reinterpret_cast<const double*>(&beta), reinterpret_cast<double*>(c), ldc);
+    else if constexpr (std::is_same<T, float>::value)
       return pb_sgemm(p, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc);
     else
       return pb_dgemm(p, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc);

@@ -53,6 +53,8 @@ SIGNATURES: dict[str, list] = {
     "pb_gemm_grouped_plan_run": [_p, _p],
     "pb_gemm_grouped_plan_destroy": [_p],
     "pb_gemm_grouped": [_p, _i32, _p, _i64],
+    "pb_cgemm": [_p, _ch, _ch, _i64, _i64, _i64, _p, _p, _i64, _p, _i64, _p, _p, _i64],
+    "pb_zgemm": [_p, _ch, _ch, _i64, _i64, _i64, _p, _p, _i64, _p, _i64, _p, _p, _i64],
     "pb_ssymm": [_p, _ch, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
     "pb_dsymm": [_p, _ch, _ch, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64],
     "pb_strsm": [_p, _ch, _ch, _ch, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64],

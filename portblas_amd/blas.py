@@ -162,6 +162,15 @@ class SB_Handle:
 # ---------------------------------------------------------------------------
 def _gemm(sb_handle, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc):
     L = sb_handle._L
+    dt = str(getattr(c, "dtype", ""))
+    if "complex" in dt:  # interleaved complex64 / complex128 tensors
+        real = ctypes.c_double if "128" in dt else ctypes.c_float
+        fn = L.pb_zgemm if "128" in dt else L.pb_cgemm
+        al = (real * 2)(complex(alpha).real, complex(alpha).imag)
+        be = (real * 2)(complex(beta).real, complex(beta).imag)
+        check(fn(sb_handle.handle, _t(transa), _t(transb), M, N, K, al, _addr(a), lda, _addr(b), ldb, be,
+                 _addr(c), ldc))
+        return
     k = _kind(c)
     fn = L.pb_dgemm if k == "f64" else L.pb_sgemm
     check(fn(sb_handle.handle, _t(transa), _t(transb), M, N, K, alpha, _addr(a), lda, _addr(b), ldb,

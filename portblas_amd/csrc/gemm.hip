@@ -494,11 +494,146 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
   return PB_STATUS_SUCCESS;
 }
 
+// ---------------------------------------------------------------------------
+// Complex GEMM (the reference's _gemm on complex types, BLAS_ENABLE_COMPLEX:
+// src/interface/gemm_interface.hpp:58-64,140-198; 'c' is taken as 't', as
+// the reference does, without conjugation).  Interleaved (re, im) operands
+// are packed into one real problem of twice the size,
+//   [T_re]   [op(A)_re  -op(A)_im] [op(B)_re]
+//   [T_im] = [op(A)_im   op(A)_re] [op(B)_im]      (2M x 2K times 2K x N),
+// which one MFMA GEMM solves with K-contiguous-friendly unit strides; an
+// epilogue then forms C <- alpha T + beta C in complex arithmetic (beta == 0
+// never reads C).  The 4 real products of the textbook method are all
+// there, summed over 2K terms in FP32 / FP64.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void cplx_pack_a_kernel(int64_t M, int64_t K, const T* __restrict__ a, int64_t rs, int64_t ks,
+                                   T* __restrict__ ahat) {
+  const int64_t total = M * K, ld = 2 * M;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = e % M, k = e / M;
+    const int64_t src = 2 * (i * rs + k * ks);
+    const T re = a[src], im = a[src + 1];
+    ahat[i + k * ld] = re;
+    ahat[i + (K + k) * ld] = -im;
+    ahat[(M + i) + k * ld] = im;
+    ahat[(M + i) + (K + k) * ld] = re;
+  }
+}
+
+template <typename T>
+__global__ void cplx_pack_b_kernel(int64_t K, int64_t N, const T* __restrict__ b, int64_t ks, int64_t ns,
+                                   T* __restrict__ bhat) {
+  const int64_t total = K * N, ld = 2 * K;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t k = e % K, j = e / K;
+    const int64_t src = 2 * (k * ks + j * ns);
+    bhat[k + j * ld] = b[src];
+    bhat[(K + k) + j * ld] = b[src + 1];
+  }
+}
+
+// C <- alpha T + beta C, T = [T_re; T_im] (2M x N, ld 2M); beta == 0: C not
+// read (beta_zero), alpha == 0: T not read (C <- beta C).
+template <typename T>
+__global__ void cplx_epilogue_kernel(int64_t M, int64_t N, T ar, T ai, T br, T bi, const T* __restrict__ t,
+                                     T* __restrict__ c, int64_t ldc, bool beta_zero, bool alpha_zero) {
+  const int64_t total = M * N;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t i = e % M, j = e / M;
+    T re = T(0), im = T(0);
+    if (!alpha_zero) {
+      const T tr = t[i + j * 2 * M], ti = t[M + i + j * 2 * M];
+      re = ar * tr - ai * ti;
+      im = ar * ti + ai * tr;
+    }
+    T* cp = c + 2 * (i + j * ldc);
+    if (!beta_zero) {
+      const T cr = cp[0], ci = cp[1];
+      re += br * cr - bi * ci;
+      im += br * ci + bi * cr;
+    }
+    cp[0] = re;
+    cp[1] = im;
+  }
+}
+
+template <typename T>
+pb_status_t cgemm_entry(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64_t K, const T* alpha,
+                        const T* a, int64_t lda, const T* b, int64_t ldb, const T* beta, T* c, int64_t ldc) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  PB_REQUIRE(alpha != nullptr && beta != nullptr, "null scalar pointer");
+  const char nta = pb::norm_trans(ta), ntb = pb::norm_trans(tb);
+  PB_REQUIRE(nta, "invalid _TransA");
+  PB_REQUIRE(ntb, "invalid _TransB");
+  PB_REQUIRE(M >= 0 && N >= 0 && K >= 0, "negative dimension");
+  if (M == 0 || N == 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(ldc >= M && lda >= (nta == 'n' ? M : K) && ldb >= (ntb == 'n' ? K : N) && lda >= 1 && ldb >= 1,
+             "leading dimension too small");
+  PB_REQUIRE(c != nullptr, "null C");
+  const bool alpha_zero = alpha[0] == T(0) && alpha[1] == T(0);
+  const bool beta_zero = beta[0] == T(0) && beta[1] == T(0);
+  int64_t g = pb::ceil_div(M * N, 256);
+  if (g > 8 * h->num_cus) g = 8 * h->num_cus;
+  if (alpha_zero || K == 0) {
+    hipLaunchKernelGGL(cplx_epilogue_kernel<T>, dim3(unsigned(g)), dim3(256), 0, h->stream, M, N, T(0), T(0),
+                       beta[0], beta[1], static_cast<const T*>(nullptr), c, ldc, beta_zero, true);
+    PB_CHECK_LAUNCH();
+    return PB_STATUS_SUCCESS;
+  }
+  PB_REQUIRE(a != nullptr && b != nullptr, "null matrix pointer");
+  GemmCall<T> gc;
+  gc.ta = 'n'; gc.tb = 'n';
+  gc.M = 2 * M; gc.N = N; gc.K = 2 * K;
+  gc.alpha = T(1); gc.beta = T(0);
+  const size_t part = pb::align_up(plain_partial_bytes<T>(h, gc), 256);
+  const size_t ahat_b = pb::align_up(size_t(4) * M * K * sizeof(T), 256);
+  const size_t bhat_b = pb::align_up(size_t(2) * K * N * sizeof(T), 256);
+  void* ws = nullptr;
+  PB_TRY(pb::workspace(h, part + ahat_b + bhat_b + size_t(2) * M * N * sizeof(T), &ws));
+  char* base = static_cast<char*>(ws);
+  T* ahat = reinterpret_cast<T*>(base + part);
+  T* bhat = reinterpret_cast<T*>(base + part + ahat_b);
+  T* that = reinterpret_cast<T*>(base + part + ahat_b + bhat_b);
+  // op(A)(i, k) at a[2 (i rs + k ks)], op(B)(k, j) at b[2 (k ks + j ns)] (complex elements).
+  const int64_t a_rs = nta == 'n' ? 1 : lda, a_ks = nta == 'n' ? lda : 1;
+  const int64_t b_ks = ntb == 'n' ? 1 : ldb, b_ns = ntb == 'n' ? ldb : 1;
+  int64_t ga = std::min<int64_t>(pb::ceil_div(M * K, 256), 8 * h->num_cus);
+  int64_t gb = std::min<int64_t>(pb::ceil_div(K * N, 256), 8 * h->num_cus);
+  hipLaunchKernelGGL(cplx_pack_a_kernel<T>, dim3(unsigned(ga)), dim3(256), 0, h->stream, M, K, a, a_rs, a_ks, ahat);
+  hipLaunchKernelGGL(cplx_pack_b_kernel<T>, dim3(unsigned(gb)), dim3(256), 0, h->stream, K, N, b, b_ks, b_ns, bhat);
+  PB_CHECK_LAUNCH();
+  gc.a = ahat; gc.lda = 2 * M;
+  gc.b = bhat; gc.ldb = 2 * K;
+  gc.c = that; gc.ldc = 2 * M;
+  PB_TRY(run_plain<T>(h, gc, false, part ? static_cast<T*>(ws) : nullptr));
+  hipLaunchKernelGGL(cplx_epilogue_kernel<T>, dim3(unsigned(g)), dim3(256), 0, h->stream, M, N, alpha[0],
+                     alpha[1], beta[0], beta[1], static_cast<const T*>(that), c, ldc, beta_zero, false);
+  PB_CHECK_LAUNCH();
+  return PB_STATUS_SUCCESS;
+}
+
 }  // namespace pbgemm
 
 using namespace pbgemm;
 
 extern "C" {
+
+pb_status_t pb_cgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, const float* alpha,
+                     const float* a, int64_t lda, const float* b, int64_t ldb, const float* beta, float* c,
+                     int64_t ldc) {
+  return cgemm_entry<float>(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
+
+pb_status_t pb_zgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, const double* alpha,
+                     const double* a, int64_t lda, const double* b, int64_t ldb, const double* beta, double* c,
+                     int64_t ldc) {
+  return cgemm_entry<double>(h, ta, tb, m, n, k, alpha, a, lda, b, ldb, beta, c, ldc);
+}
 
 pb_status_t pb_strsm(pb_handle_t h, char side, char uplo, char trans, char diag, int64_t m, int64_t n,
                      float alpha, const float* a, int64_t lda, float* b, int64_t ldb) {
