@@ -293,6 +293,22 @@ pb_status_t pb_isamin_sync(pb_handle_t h, int64_t n, const float* x, int64_t inc
                            int32_t* result_host);
 
 /* ------------------------------------------------------------------------ */
+/* FP16 GEMMs: the reference's _gemm on sycl::half (BLAS_ENABLE_HALF;       */
+/* src/interface/blas3/backend/default.hpp:150-246, amd_gpu.hpp:292-359).    */
+/* Operands are IEEE binary16 bit patterns, accumulation is FP32 (MFMA).     */
+/*   pb_hgemm:    A, B, C half; alpha and beta passed as float; the result   */
+/*                is rounded to half once (beta == 0 never reads C).         */
+/*   pb_gemm_f16: A, B half, C FP32 (strided batches), like pb_gemm_bf16.    */
+/* ------------------------------------------------------------------------ */
+pb_status_t pb_hgemm(pb_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                     float alpha, const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta,
+                     uint16_t* c, int64_t ldc);
+pb_status_t pb_gemm_f16(pb_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                        float alpha, const uint16_t* a, int64_t lda, int64_t stridea, const uint16_t* b,
+                        int64_t ldb, int64_t strideb, float beta, float* c, int64_t ldc, int64_t stridec,
+                        int64_t batch);
+
+/* ------------------------------------------------------------------------ */
 /* Complex GEMM: blas::_gemm on std::complex<float / double> (built with     */
 /* BLAS_ENABLE_COMPLEX; src/interface/gemm_interface.hpp:58-64,140-198,     */
 /* test/unittest/blas3/blas3_gemm_common.hpp:480-580).  Matrices hold        */

@@ -22,6 +22,7 @@ from .blas import (  # noqa: F401
     _gemm_batched,
     _gemm_bf16,
     _gemm_ex,
+    _gemm_f16,
     _gemm_strided_batched,
     _gemv,
     _iamax,

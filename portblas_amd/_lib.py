@@ -53,6 +53,9 @@ SIGNATURES: dict[str, list] = {
     "pb_gemm_grouped_plan_run": [_p, _p],
     "pb_gemm_grouped_plan_destroy": [_p],
     "pb_gemm_grouped": [_p, _i32, _p, _i64],
+    "pb_hgemm": [_p, _ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
+    "pb_gemm_f16": [_p, _ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _i64, _p, _i64, _i64, _f32, _p,
+                    _i64, _i64, _i64],
     "pb_cgemm": [_p, _ch, _ch, _i64, _i64, _i64, _p, _p, _i64, _p, _i64, _p, _p, _i64],
     "pb_zgemm": [_p, _ch, _ch, _i64, _i64, _i64, _p, _p, _i64, _p, _i64, _p, _p, _i64],
     "pb_ssymm": [_p, _ch, _ch, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],

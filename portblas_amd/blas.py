@@ -163,6 +163,10 @@ class SB_Handle:
 def _gemm(sb_handle, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc):
     L = sb_handle._L
     dt = str(getattr(c, "dtype", ""))
+    if dt.endswith("float16") and "b" not in dt.split(".")[-1]:  # torch.float16 / numpy float16
+        check(L.pb_hgemm(sb_handle.handle, _t(transa), _t(transb), M, N, K, float(alpha), _addr(a), lda,
+                         _addr(b), ldb, float(beta), _addr(c), ldc))
+        return
     if "complex" in dt:  # interleaved complex64 / complex128 tensors
         real = ctypes.c_double if "128" in dt else ctypes.c_float
         fn = L.pb_zgemm if "128" in dt else L.pb_cgemm
@@ -199,6 +203,13 @@ def _gemm_ex(sb_handle, precision, transa, transb, M, N, K, alpha, a, lda, b, ld
     check(sb_handle._L.pb_gemm_ex(sb_handle.handle, int(precision), _t(transa), _t(transb), M, N, K,
                                   alpha, _addr(a), lda, stridea, _addr(b), ldb, strideb, beta,
                                   _addr(c), ldc, stridec, batch_size))
+
+
+def _gemm_f16(sb_handle, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc,
+              stridea=0, strideb=0, stridec=0, batch_size=1):
+    """FP16 inputs, FP32 accumulate and output."""
+    check(sb_handle._L.pb_gemm_f16(sb_handle.handle, _t(transa), _t(transb), M, N, K, alpha, _addr(a), lda,
+                                   stridea, _addr(b), ldb, strideb, beta, _addr(c), ldc, stridec, batch_size))
 
 
 def _gemm_bf16(sb_handle, transa, transb, M, N, K, alpha, a, lda, b, ldb, beta, c, ldc,

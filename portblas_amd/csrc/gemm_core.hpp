@@ -20,6 +20,7 @@
 //   float  : v_mfma_f32_16x16x4_f32   (FP32 in, FP32 accumulate)
 //   double : v_mfma_f64_16x16x4_f64   (FP64 in, FP64 accumulate)
 //   bf16   : v_mfma_f32_32x32x16_bf16 (BF16 in, FP32 accumulate)
+//   fp16   : v_mfma_f32_32x32x16_f16  (FP16 in, FP32 accumulate)
 //   fp8    : v_mfma_scale_f32_16x16x128_f8f6f4, e4m3 operands, unit scales
 //
 // The reference kernel this replaces is Gemm<local, standard, full>
@@ -40,6 +41,7 @@ typedef int i32x8 __attribute__((ext_vector_type(8)));
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));  // 16-byte staging unit
 
 struct bf16_t { uint16_t bits; };
+struct fp16_t { uint16_t bits; };  // IEEE binary16
 struct fp8_t { uint8_t bits; };
 
 // ---------------------------------------------------------------------------
@@ -68,6 +70,16 @@ template <> struct Traits<double> {
   static constexpr int NACC = 4;
 };
 template <> struct Traits<bf16_t> {
+  using Raw = uint16_t;
+  using Acc = float;
+  using AccV = f32x16;
+  static constexpr int VW = 8;
+  static constexpr int MR = 32;
+  static constexpr int KG = 16;
+  static constexpr int KPL = 8;
+  static constexpr int NACC = 16;
+};
+template <> struct Traits<fp16_t> {  // same fragment shape as BF16 (32x32x16)
   using Raw = uint16_t;
   using Acc = float;
   using AccV = f32x16;
@@ -126,6 +138,15 @@ __device__ __forceinline__ void mma_group<bf16_t>(f32x16& acc, const Frag<bf16_t
   __builtin_memcpy(&av, a.v, 16);
   __builtin_memcpy(&bv, b.v, 16);
   acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(av, bv, acc, 0, 0, 0);
+}
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+template <>
+__device__ __forceinline__ void mma_group<fp16_t>(f32x16& acc, const Frag<fp16_t>& a,
+                                                  const Frag<fp16_t>& b) {
+  f16x8 av, bv;
+  __builtin_memcpy(&av, a.v, 16);
+  __builtin_memcpy(&bv, b.v, 16);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_f16(av, bv, acc, 0, 0, 0);
 }
 template <>
 __device__ __forceinline__ void mma_group<fp8_t>(f32x4& acc, const Frag<fp8_t>& a,

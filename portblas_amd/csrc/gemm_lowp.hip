@@ -535,6 +535,48 @@ pb_status_t gemm_bf16_entry(pb_handle_t h, int id, char ta, char tb, int64_t m, 
   return gemm_bf16_direct(h, id, g, alpha, beta, c);
 }
 
+// ---------------------------------------------------------------------------
+// FP16 GEMMs (the reference's _gemm on sycl::half, BLAS_ENABLE_HALF:
+// src/interface/blas3/backend/default.hpp:150-246, amd_gpu.hpp:292-359).  FP16
+// inputs through v_mfma_f32_32x32x16_f16 with FP32 accumulation; two of the
+// BF16 engine's tile shapes (128 x 128 for problems that fill the GPU with
+// them, else 64 x 64 with split-K), chosen like the FP32 plan.
+// ---------------------------------------------------------------------------
+pb_status_t gemm_f16_direct(pb_handle_t h, const GemmCall<uint16_t>& g, float alpha, float beta, float* c,
+                            int64_t ldc, int64_t sc) {
+  GemmProblem<fp16_t> p{};
+  p.M = g.M; p.N = g.N; p.K = g.K; p.alpha = alpha; p.beta = beta;
+  p.nterms = 1;
+  p.a[0] = g.a;
+  p.b[0] = g.b;
+  p.batch = g.batch;
+  const bool vec = set_operands(p, g.ta, g.tb, g.lda, g.sa, g.ldb, g.sb, false);
+  p.c = c; p.c_rs = 1; p.ldc = ldc; p.c_stride = sc;
+  p.c_vec = aligned16(c) && ldc % 4 == 0 && (g.batch == 1 || sc % 4 == 0);
+  const int64_t cus = h->num_cus;
+  const int64_t t128 = pb::ceil_div(g.M, 128) * pb::ceil_div(g.N, 128) * g.batch;
+  const int64_t t64 = pb::ceil_div(g.M, 64) * pb::ceil_div(g.N, 64) * g.batch;
+  if (t128 >= 2 * cus) return launch_gemm<fp16_t, B128>(h, p, g.ta == 't', g.tb == 'n', vec, 1);
+  int splits = 1;
+  while (t64 * splits < 2 * cus && g.K / (splits * 2) >= 256 && splits < 16) splits *= 2;
+  return launch_gemm<fp16_t, B64>(h, p, g.ta == 't', g.tb == 'n', vec, splits);
+}
+
+// C (half) <- half(T + beta * C) with T already alpha * op(A) op(B) in FP32
+// (t == nullptr: T = 0); beta == 0 never reads C.
+__global__ void f16_store_kernel(int64_t M, int64_t N, const float* __restrict__ t, int64_t ldt,
+                                 int64_t st, float beta, _Float16* c, int64_t ldc, int64_t sc, int64_t batch) {
+  const int64_t per = M * N, total = per * batch;
+  for (int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < total;
+       e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t b = e / per, r = e % per, i = r % M, j = r / M;
+    _Float16* cp = c + b * sc + i + j * ldc;
+    float v = t ? t[b * st + i + j * ldt] : 0.f;
+    if (beta != 0.f) v += beta * float(*cp);
+    *cp = _Float16(v);
+  }
+}
+
 }  // namespace pbgemm
 
 using namespace pbgemm;
@@ -683,6 +725,51 @@ pb_status_t pb_tune_time_config(pb_handle_t h, int id, char ta, char tb, int64_t
   if (e != hipSuccess)
     return pb::set_error(PB_STATUS_DEVICE_ERROR, "timing failed: %s", hipGetErrorString(e));
   *usec_out = 1000.f * ms / float(reps);
+  return PB_STATUS_SUCCESS;
+}
+
+// FP16 inputs, FP32 accumulate and output (the analogue of pb_gemm_bf16).
+pb_status_t pb_gemm_f16(pb_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                        float alpha, const uint16_t* a, int64_t lda, int64_t stridea, const uint16_t* b,
+                        int64_t ldb, int64_t strideb, float beta, float* c, int64_t ldc, int64_t stridec,
+                        int64_t batch) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  GemmCall<uint16_t> g;
+  bool done = false;
+  PB_TRY((gemm_front<float, uint16_t>(h, transa, transb, m, n, k, alpha, a, lda, stridea, b, ldb, strideb,
+                                      beta, c, ldc, stridec, batch, PB_BATCH_STRIDED, g, done)));
+  if (done) return PB_STATUS_SUCCESS;
+  if (alpha == 0.f) return sgemm_alpha_zero(h, m, n, beta, c, ldc, stridec, batch);
+  return gemm_f16_direct(h, g, alpha, beta, c, ldc, stridec);
+}
+
+// All-FP16 GEMM (A, B, C half; alpha and beta given as float): the FP32
+// product alpha op(A) op(B) goes to the workspace, then C <- half(T + beta C).
+pb_status_t pb_hgemm(pb_handle_t h, char transa, char transb, int64_t m, int64_t n, int64_t k,
+                     float alpha, const uint16_t* a, int64_t lda, const uint16_t* b, int64_t ldb, float beta,
+                     uint16_t* c, int64_t ldc) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  GemmCall<uint16_t> g;
+  bool done = false;
+  // Validation with C typed as the FP32 product (ld M) that is written first.
+  PB_TRY((gemm_front<float, uint16_t>(h, transa, transb, m, n, k, alpha, a, lda, 0, b, ldb, 0, beta,
+                                      reinterpret_cast<float*>(c), ldc, 0, 1, PB_BATCH_STRIDED, g, done)));
+  if (done) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(c != nullptr, "null C");
+  // Workspace: split-K partials (64 x 64 shape at most 16 splits), then T.
+  const size_t part = pb::align_up(splitk_bytes<fp16_t, B64>(m, n, k, 1, 16), 256);
+  void* ws = nullptr;
+  PB_TRY(pb::workspace(h, part + size_t(m) * size_t(n) * sizeof(float), &ws));
+  float* t = reinterpret_cast<float*>(static_cast<char*>(ws) + part);
+  const bool no_product = alpha == 0.f || k == 0;
+  if (!no_product) PB_TRY(gemm_f16_direct(h, g, alpha, 0.f, t, m, m * n));
+  int64_t grid = std::min<int64_t>(pb::ceil_div(m * n, 256), 8 * h->num_cus);
+  hipLaunchKernelGGL(f16_store_kernel, dim3(unsigned(std::max<int64_t>(grid, 1))), dim3(256), 0, h->stream, m,
+                     n, no_product ? static_cast<const float*>(nullptr) : static_cast<const float*>(t), m,
+                     m * n, beta, reinterpret_cast<_Float16*>(c), ldc, ldc * n, int64_t(1));
+  PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }
 
