@@ -61,6 +61,57 @@ __global__ void amax_kernel(const float* __restrict__ x, int64_t rs, int64_t ks,
   if ((threadIdx.x & 63) == 0) atomicMax(out, best);
 }
 
+// amax of a dense block of n floats (an operand stored without padding):
+// 16-byte non-temporal loads, 4 in flight per lane.  NaN never wins (fmaxf
+// returns the other operand); the bits of a non-negative float order like
+// the float, so the workgroup results combine with an unsigned atomicMax.
+__global__ __launch_bounds__(256) void amax_flat_kernel(const float* __restrict__ x, int64_t n,
+                                                        unsigned int* out) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const int64_t nvec = n / 4;
+  const f4* xv = reinterpret_cast<const f4*>(x);
+  float best = 0.f;
+  const int64_t step = int64_t(gridDim.x) * 256 * 4;
+  for (int64_t base = int64_t(blockIdx.x) * 256 * 4 + threadIdx.x; base < nvec; base += step) {
+    f4 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = base + u * 256 < nvec ? __builtin_nontemporal_load(xv + base + u * 256) : f4{0, 0, 0, 0};
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      best = fmaxf(best, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
+  }
+  if (blockIdx.x == 0 && threadIdx.x < n - nvec * 4) best = fmaxf(best, fabsf(x[nvec * 4 + threadIdx.x]));
+  unsigned int ub;
+  __builtin_memcpy(&ub, &best, 4);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned int o = __shfl_xor(ub, off, 64);
+    ub = o > ub ? o : ub;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, ub);
+}
+
+// amax over `nlines` lines (per batch) of `len` contiguous floats, line j of
+// batch b at x + b * stride + j * pitch: one workgroup per line at a time,
+// coalesced along the line, no per-element index arithmetic.
+__global__ __launch_bounds__(256) void amax_lines_kernel(const float* __restrict__ x, int64_t len,
+                                                         int64_t pitch, int64_t nlines, int64_t stride,
+                                                         int64_t batch, unsigned int* out) {
+  float best = 0.f;
+  for (int64_t l = blockIdx.x; l < nlines * batch; l += gridDim.x) {
+    const float* line = x + (l / nlines) * stride + (l % nlines) * pitch;
+    for (int64_t i = threadIdx.x; i < len; i += 256) best = fmaxf(best, fabsf(line[i]));
+  }
+  unsigned int ub;
+  __builtin_memcpy(&ub, &best, 4);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const unsigned int o = __shfl_xor(ub, off, 64);
+    ub = o > ub ? o : ub;
+  }
+  if ((threadIdx.x & 63) == 0) atomicMax(out, ub);
+}
+
 // scales[0] = 448 / amax_a, scales[1] = 448 / amax_b, scales[2] = the
 // epilogue factor amax_a * amax_b / 448^2 (1 for an all-zero operand).
 __global__ void fp8_scales_kernel(const unsigned int* amax_bits, float* scales) {
@@ -391,12 +442,28 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   if (fp8) {
     // amax of each operand over all batches, then the scales (device side).
     hipLaunchKernelGGL(fill_zero_u32, dim3(1), dim3(64), 0, h->stream, amax_bits, 2);
-    const int64_t ga = std::min<int64_t>(pb::ceil_div(M * K * batch, 256), 4 * h->num_cus);
-    const int64_t gb = std::min<int64_t>(pb::ceil_div(N * K * batch, 256), 4 * h->num_cus);
-    hipLaunchKernelGGL(amax_kernel, dim3(unsigned(std::max<int64_t>(ga, 1))), dim3(256), 0,
-                       h->stream, g.a, a_rs, a_ks, M, K, g.sa, batch, amax_bits);
-    hipLaunchKernelGGL(amax_kernel, dim3(unsigned(std::max<int64_t>(gb, 1))), dim3(256), 0,
-                       h->stream, g.b, b_rs, b_ks, N, K, g.sb, batch, amax_bits + 1);
+    // Dense operands stream as one block; others go line by line along their
+    // contiguous dimension.
+    auto amax = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, unsigned int* out) {
+      const int64_t per = rows * K;
+      const bool dense = ((rs == 1 && ks == rows) || (ks == 1 && rs == K)) && (batch == 1 || stride == per);
+      if (dense && aligned16(x)) {
+        const int64_t gv = std::min<int64_t>(pb::ceil_div(per * batch, 4096), 8 * h->num_cus);
+        hipLaunchKernelGGL(amax_flat_kernel, dim3(unsigned(std::max<int64_t>(gv, 1))), dim3(256), 0, h->stream, x,
+                           per * batch, out);
+      } else if (rs == 1 || ks == 1) {
+        const int64_t len = rs == 1 ? rows : K, nl = rs == 1 ? K : rows, pitch = rs == 1 ? ks : rs;
+        const int64_t gl = std::min<int64_t>(nl * batch, 8 * h->num_cus);
+        hipLaunchKernelGGL(amax_lines_kernel, dim3(unsigned(std::max<int64_t>(gl, 1))), dim3(256), 0, h->stream,
+                           x, len, pitch, nl, stride, batch, out);
+      } else {
+        const int64_t ge = std::min<int64_t>(pb::ceil_div(per * batch, 256), 4 * h->num_cus);
+        hipLaunchKernelGGL(amax_kernel, dim3(unsigned(std::max<int64_t>(ge, 1))), dim3(256), 0, h->stream, x, rs,
+                           ks, rows, K, stride, batch, out);
+      }
+    };
+    amax(g.a, a_rs, a_ks, M, g.sa, amax_bits);
+    amax(g.b, b_rs, b_ks, N, g.sb, amax_bits + 1);
     hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(1), 0, h->stream, amax_bits, scales);
     PB_CHECK_LAUNCH();
   }

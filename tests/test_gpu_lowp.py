@@ -38,13 +38,13 @@ def make(torch, n, seed, dist="uniform"):
 
 
 def check_lowp(handle, torch, prec, ta, tb, m, n, k, alpha=1.0, beta=0.0, batch=1, seed=0,
-               dist="uniform"):
+               dist="uniform", pad=0):
     import portblas_amd as pb
 
     P = pb.gemm_precision_t
     ar, ac = (m, k) if ta == "n" else (k, m)
     br, bc = (k, n) if tb == "n" else (n, k)
-    lda, ldb, ldc = max(ar, 1), max(br, 1), max(m, 1)
+    lda, ldb, ldc = max(ar, 1) + pad, max(br, 1) + pad, max(m, 1)
     sa, sb, sc = lda * ac, ldb * bc, ldc * n
     A = make(torch, sa * batch, seed * 3 + 1, dist)
     B = make(torch, sb * batch, seed * 3 + 2, dist)
@@ -104,6 +104,18 @@ def test_lowp_large_tiles(handle, torch_cuda, prec, ta, tb, m, n, k, batch):
 
     check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, m, n, k, -1.25, 0.75, batch=batch,
                seed=3)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("ta,tb", [("n", "t"), ("t", "n")])
+def test_lowp_padded_leading_dimensions(handle, torch_cuda, prec, ta, tb):
+    """Operands with lda / ldb above their row count (the pre-passes walk them
+    line by line rather than as one dense block)."""
+    import portblas_amd as pb
+
+    check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, 300, 260, 200, 1.5, 0.5, pad=5, seed=8)
+    check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, 2048, 1024, 136, 1.0, 0.0, batch=2, pad=3,
+               seed=9)
 
 
 @pytest.mark.parametrize("dist", ["normal", "loguniform"])
