@@ -94,6 +94,14 @@ def test_gemv_unaligned_and_double(handle, torch_cuda, tr):
     _gemv_check(handle, torch_cuda, tr, 1000, 777, -1.0, 0.25, 1004, 1, 1, seed=11, dtype=np.float64)
 
 
+def test_gemv_t_row_chunks(handle, torch_cuda):
+    """'t' with M around and beyond a few 2048-row chunks (partial last
+    chunk, several chunks folded), FP32 and FP64."""
+    for m, dt in ((8192, np.float32), (8195, np.float32), (20000, np.float32), (4096, np.float64),
+                  (5000, np.float64)):
+        _gemv_check(handle, torch_cuda, "t", m, 130, 1.25, 0.5, m, 1, 1, seed=m, dtype=dt)
+
+
 @pytest.mark.parametrize("tr", ["n", "t"])
 def test_gemv_full_size_rows(handle, torch_cuda, tr):
     """BASELINE's 8192 x 8192 SGEMV, alpha 1 beta 0; 64 output entries are
