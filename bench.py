@@ -51,9 +51,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--quick", action="store_true", help="headline + roofline only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--pmc-probe", action="store_true",
+    ap.add_argument("--pmc-probe", nargs="?", const="bf16x3", choices=("bf16x3", "dgemm"), default=None,
                     help="for rocprofv3 --pmc: one 256 MiB device copy (the counter scale), then two "
-                         "launches of the BF16x3 kernel on packed 8192^3 operands; prints no JSON")
+                         "launches of the BF16x3 kernel on packed 8192^3 operands (or of DGEMM 4096^3); "
+                         "prints no JSON")
     return ap.parse_args()
 
 
@@ -96,15 +97,26 @@ def committed_traffic():
     return float(m.group(1)) if m else None
 
 
-def pmc_probe(torch, pb, h):
+def pmc_probe(torch, pb, h, which="bf16x3"):
     """A plain 256 MiB copy, whose FETCH_SIZE / WRITE_SIZE give the counters'
-    scale, then the BF16x3 kernel alone on packed 8192^3 operands, twice."""
+    scale, then the BF16x3 kernel alone on packed 8192^3 operands (or
+    DGEMM 4096^3), twice."""
     src = torch.ones(1 << 26, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
     dst.copy_(src)
     torch.cuda.synchronize()
     del src, dst
+    if which == "dgemm":
+        D = 4096
+        Ad = torch.rand(D * D, dtype=torch.float64, device="cuda")
+        Bd = torch.rand(D * D, dtype=torch.float64, device="cuda")
+        Cd = torch.zeros(D * D, dtype=torch.float64, device="cuda")
+        for _ in range(2):
+            pb._gemm(h, "n", "n", D, D, D, 1.0, Ad, D, Bd, D, 0.0, Cd, D)
+            torch.cuda.synchronize()
+        log("pmc probe done")
+        return
     L = pb.load()
     N = 8192
     A = torch.randn(N * N, device="cuda")
@@ -179,7 +191,7 @@ def main():
     torch.cuda.set_device(0)
     h = pb.SB_Handle()
     if args.pmc_probe:
-        pmc_probe(torch, pb, h)
+        pmc_probe(torch, pb, h, args.pmc_probe)
         return
     timer = Timer(torch)
     res: dict = {}
