@@ -233,31 +233,50 @@ struct Tile {
   }
 
 
+  // 16-byte load through a global-address-space pointer: a global_load
+  // counts against vmcnt only, so the LDS-read waits before the MFMAs do not
+  // also wait for the next K tile's loads (a flat load would count on both).
+  static __device__ __forceinline__ u32x4 ld_global16(const Raw* p) {
+    typedef const __attribute__((address_space(1))) u32x4 gvec;
+    return *(gvec*)(p);
+  }
+
   // Fast path: 16-byte loads from a block base pointer (interior tile, K
-  // tile complete, operands vector-aligned).
+  // tile complete, operands vector-aligned).  A K-contiguous block arrives
+  // as VW rows; an M/N-contiguous one as VW columns (out[j] = elements
+  // (r0 .. r0+VW-1, k0 + j)), transposed only when it is written to LDS so
+  // that nothing waits for the loads before the current tile's MFMAs.
   template <bool KC>
   static __device__ __forceinline__ void load_fast(const Raw* p, int64_t rs, int64_t ks,
                                                    u32x4 out[VW]) {
     if constexpr (KC) {
 #pragma unroll
       for (int i = 0; i < VW; ++i) {
-        out[i] = *reinterpret_cast<const u32x4*>(p);
+        out[i] = ld_global16(p);
         p += rs;
       }
     } else {
+#pragma unroll
+      for (int j = 0; j < VW; ++j) out[j] = ld_global16(p + j * ks);
+    }
+  }
+
+  // Row i of a block: as loaded (K-contiguous) or gathered from the VW
+  // loaded columns.
+  template <bool KC>
+  static __device__ __forceinline__ u32x4 block_row(const u32x4 (&v)[VW], int i) {
+    if constexpr (KC) {
+      return v[i];
+    } else {
       Raw col[VW][VW];  // col[j][i] = element (r0 + i, k0 + j)
 #pragma unroll
-      for (int j = 0; j < VW; ++j) {
-        const u32x4 v = *reinterpret_cast<const u32x4*>(p + j * ks);
-        __builtin_memcpy(col[j], &v, 16);
-      }
+      for (int j = 0; j < VW; ++j) __builtin_memcpy(col[j], &v[j], 16);
+      Raw row[VW];
 #pragma unroll
-      for (int i = 0; i < VW; ++i) {
-        Raw row[VW];
-#pragma unroll
-        for (int j = 0; j < VW; ++j) row[j] = col[j][i];
-        __builtin_memcpy(&out[i], row, 16);
-      }
+      for (int j = 0; j < VW; ++j) row[j] = col[j][i];
+      u32x4 out;
+      __builtin_memcpy(&out, row, 16);
+      return out;
     }
   }
 
@@ -308,7 +327,7 @@ struct Tile {
         block_coords<KCA, BM>(blk, rb, kb);
 #pragma unroll
         for (int i = 0; i < VW; ++i)
-          *reinterpret_cast<u32x4*>(lds + (rb * VW + i) * LDK + kb * VW) = st.a[t][i];
+          *reinterpret_cast<u32x4*>(lds + (rb * VW + i) * LDK + kb * VW) = block_row<KCA>(st.a[t], i);
       }
     }
 #pragma unroll
@@ -319,7 +338,8 @@ struct Tile {
         block_coords<KCB, BN>(blk, rb, kb);
 #pragma unroll
         for (int i = 0; i < VW; ++i)
-          *reinterpret_cast<u32x4*>(lds + A_ELEMS + (rb * VW + i) * LDK + kb * VW) = st.b[t][i];
+          *reinterpret_cast<u32x4*>(lds + A_ELEMS + (rb * VW + i) * LDK + kb * VW) =
+              block_row<KCB>(st.b[t], i);
       }
     }
   }
