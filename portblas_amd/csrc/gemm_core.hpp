@@ -353,6 +353,36 @@ struct Tile {
     const int g = lane / MR;
     const Raw* As = lds;
     const Raw* Bs = lds + A_ELEMS;
+    if constexpr (sizeof(Acc) == 8) {
+      // FP64: a lane's 4 k values per group are read as two 16-byte halves
+      // (2 MFMA steps each), which halves the fragment registers; each
+      // accumulator still sees its k steps in the order 0, 1, 2, 3.
+#pragma unroll
+      for (int kg = 0; kg < BK; kg += KG) {
+#pragma unroll
+        for (int h = 0; h < KPL; h += 2) {
+          u32x4 fa[FM], fb[FN];
+#pragma unroll
+          for (int i = 0; i < FM; ++i)
+            fa[i] = *reinterpret_cast<const u32x4*>(As + (wm + i * MR + r) * LDK + kg + g * KPL + h);
+#pragma unroll
+          for (int j = 0; j < FN; ++j)
+            fb[j] = *reinterpret_cast<const u32x4*>(Bs + (wn + j * MR + r) * LDK + kg + g * KPL + h);
+#pragma unroll
+          for (int s = 0; s < 2; ++s)
+#pragma unroll
+            for (int i = 0; i < FM; ++i)
+#pragma unroll
+              for (int j = 0; j < FN; ++j) {
+                double av, bv;
+                __builtin_memcpy(&av, reinterpret_cast<const char*>(&fa[i]) + 8 * s, 8);
+                __builtin_memcpy(&bv, reinterpret_cast<const char*>(&fb[j]) + 8 * s, 8);
+                acc[i][j] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc[i][j], 0, 0, 0);
+              }
+        }
+      }
+      return;
+    }
 #pragma unroll
     for (int kg = 0; kg < BK; kg += KG) {
       Frag<T> fa[FM], fb[FN];

@@ -62,7 +62,7 @@ __device__ __forceinline__ void tile_of(const GemmProblem<T>& p, int64_t lin, in
 }
 
 template <typename T, typename Shape, bool KCA, bool KCB, bool EDGE>
-__global__ __launch_bounds__(256) void gemm_kernel(GemmProblem<T> p) {
+__global__ __launch_bounds__(256, (sizeof(T) == 8 && Shape::kBM == 128) ? 2 : 1) void gemm_kernel(GemmProblem<T> p) {
   using TL = Tile<T, Shape, KCA, KCB>;
   using Acc = typename Traits<T>::Acc;
   __shared__ __attribute__((aligned(16))) typename TL::Raw lds[2 * TL::STAGE_ELEMS];
