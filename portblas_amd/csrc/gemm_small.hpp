@@ -3,14 +3,15 @@
 // batch (src/operations/blas3/gemm_local.hpp:273-279,511-516); a 64 x 64
 // problem is one tile, and a batch of them is bound by HBM, so what matters
 // is bytes in flight: every thread issues all of its global loads for a K
-// chunk of 32 at once, the chunk goes through LDS, the next chunk's loads are
-// issued before the current chunk's MFMAs, and four waves each compute a
+// chunk at once, the chunk goes through LDS, and four waves each compute a
 // 32 x 32 quarter of C with v_mfma_f32_16x16x4_f32 (FP32) or
-// v_mfma_f64_16x16x4_f64 (FP64).  An operand keeps its contiguous dimension
-// in LDS: row-contiguous as [k][r] (pitch 80 elements, 16-byte stores),
-// K-contiguous as [r][k] (pitch 36, 16-byte stores); with 64 LDS banks both
-// make the 16-lane x 4-k MFMA fragment reads conflict-free.  10 KB (FP32) of
-// LDS per operand lets 8 FP32 (4 FP64) workgroups share a CU.
+// v_mfma_f64_16x16x4_f64 (FP64).  Full 64 x 64 FP32 problems with K a
+// multiple of 64 stage K in chunks of 64 without bounds checks (a 64^3
+// problem is one memory latency); the others use chunks of 32.  The next
+// chunk's loads are issued before the current chunk's MFMAs.  An operand keeps its contiguous dimension in LDS:
+// row-contiguous as [k][r] (pitch 80 elements, 16-byte stores), K-contiguous
+// as [r][k] (pitch KC + 4, 16-byte stores); with 64 LDS banks both make the
+// 16-lane x 4-k MFMA fragment reads conflict-free.
 #pragma once
 
 #include "gemm_launch.hpp"
@@ -19,15 +20,18 @@ namespace pbgemm {
 namespace small {
 
 constexpr int kTile = 64;    // M, N of one problem tile
-constexpr int kKc = 32;      // K chunk staged through LDS
 constexpr int kPitchR = 80;  // [k][r] layout (row-contiguous operands)
-constexpr int kPitchK = 36;  // [r][k] layout (K-contiguous operands)
-constexpr int kLdsElems = kKc * kPitchR > kTile * kPitchK ? kKc * kPitchR : kTile * kPitchK;
 
-// LDS index of element (r, k) of a staged operand.
-template <bool KC>
+// KC: K chunk staged through LDS (32, or 64 for a single-chunk FP32 problem).
+template <int KC> struct ChunkShape {
+  static constexpr int kPitchK = KC + 4;  // [r][k] layout (K-contiguous operands)
+  static constexpr int kLdsElems = KC * kPitchR > kTile * kPitchK ? KC * kPitchR : kTile * kPitchK;
+};
+
+// LDS index of element (r, k) of a staged operand ([r][k] when KCONT).
+template <bool KCONT, int KC>
 __device__ __forceinline__ int lds_at(int r, int k) {
-  return KC ? r * kPitchK + k : k * kPitchR + r;
+  return KCONT ? r * ChunkShape<KC>::kPitchK + k : k * kPitchR + r;
 }
 
 // Vector loads need 16-byte alignment of every element group they touch.
@@ -46,9 +50,11 @@ __device__ __forceinline__ void load_row_vec(const T* p, T out[16 / sizeof(T)]) 
 // rows x K) for LDS as dst[k][r]: load() issues every global load of the
 // thread, store() writes them to LDS.  VEC: 16-byte vectors along the
 // contiguous dimension (rs == 1 or ks == 1; aligned, with a pitch that is a
-// multiple of the vector width); otherwise scalar loads.
-template <typename T, bool VEC>
+// multiple of the vector width); otherwise scalar loads.  FULL (with VEC):
+// rows == 64 and the chunk lies inside K, so no load needs a bounds check.
+template <typename T, bool VEC, int KC, bool FULL = false>
 struct Chunk {
+  static constexpr int kKc = KC;
   static constexpr int VN = 16 / sizeof(T);
   static constexpr int IT = VEC ? kTile * kKc / VN / 256 : kTile * kKc / 256;
   static constexpr int W = VEC ? VN : 1;
@@ -61,7 +67,7 @@ struct Chunk {
       const int e = threadIdx.x + 256 * it;
       if (VEC && rs == 1) {  // vectors along r: (r .. r+VN-1, k)
         const int r = (e % (kTile / VN)) * VN, k = e / (kTile / VN);
-        if (k0 + k < K && r + VN <= rows) {
+        if (FULL || (k0 + k < K && r + VN <= rows)) {
           load_row_vec<T>(x + r + (k0 + k) * ks, v[it]);
         } else {
 #pragma unroll
@@ -70,7 +76,7 @@ struct Chunk {
         }
       } else if (VEC) {  // ks == 1, vectors along k: (r, k .. k+VN-1)
         const int k = (e % (kKc / VN)) * VN, r = e / (kKc / VN);
-        if (r < rows && k0 + k + VN <= K) {
+        if (FULL || (r < rows && k0 + k + VN <= K)) {
           load_row_vec<T>(x + r * rs + (k0 + k), v[it]);
         } else {
 #pragma unroll
@@ -91,14 +97,14 @@ struct Chunk {
       const int e = threadIdx.x + 256 * it;
       if (VEC && rs == 1) {
         const int r = (e % (kTile / VN)) * VN, k = e / (kTile / VN);
-        T* q = dst + lds_at<false>(r, k);
+        T* q = dst + lds_at<false, KC>(r, k);
         if constexpr (sizeof(T) == 4)
           *reinterpret_cast<pbdev::f32x4_t*>(q) = pbdev::f32x4_t{v[it][0], v[it][1], v[it][2], v[it][3]};
         else
           *reinterpret_cast<pbdev::f64x2_t*>(q) = pbdev::f64x2_t{v[it][0], v[it][1]};
       } else if (VEC) {
         const int k = (e % (kKc / VN)) * VN, r = e / (kKc / VN);
-        T* q = dst + lds_at<true>(r, k);  // 16-byte aligned (pitch 36, k a multiple of VN)
+        T* q = dst + lds_at<true, KC>(r, k);  // 16-byte aligned (pitch KC + 4, k a multiple of VN)
 #pragma unroll
         for (int i = 0; i < W; i += 8 / sizeof(T)) {
           if constexpr (sizeof(T) == 4) {
@@ -110,17 +116,19 @@ struct Chunk {
         }
       } else {
         const int r = e % kTile, k = e / kTile;
-        dst[kc ? lds_at<true>(r, k) : lds_at<false>(r, k)] = v[it][0];
+        dst[kc ? lds_at<true, KC>(r, k) : lds_at<false, KC>(r, k)] = v[it][0];
       }
     }
   }
 };
 
-template <typename T, bool VEC>
-__global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
+// FULL: M == N == 64 and K a multiple of KC (VEC only).
+template <typename T, bool VEC, int KC, bool FULL = false>
+__global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : 1) void small_gemm_kernel(GemmProblem<T> p) {
   using Acc = typename Traits<T>::AccV;  // 4 accumulators per lane (f32x4 / f64x4)
-  __shared__ __attribute__((aligned(16))) T As[kLdsElems];
-  __shared__ __attribute__((aligned(16))) T Bs[kLdsElems];
+  constexpr int kKc = KC;
+  __shared__ __attribute__((aligned(16))) T As[ChunkShape<KC>::kLdsElems];
+  __shared__ __attribute__((aligned(16))) T Bs[ChunkShape<KC>::kLdsElems];
   // Layout per operand: [r][k] when K-contiguous and vector-staged.
   const bool akc = VEC && p.a_rs != 1 && p.a_ks == 1;
   const bool bkc = VEC && p.b_rs != 1 && p.b_ks == 1;
@@ -137,7 +145,7 @@ __global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
-  Chunk<T, VEC> ca, cb;
+  Chunk<T, VEC, KC, FULL> ca, cb;
   if (p.K > 0) {
     ca.load(A, p.a_rs, p.a_ks, M, p.K, 0);
     cb.load(B, p.b_rs, p.b_ks, N, p.K, 0);
@@ -158,12 +166,12 @@ __global__ __launch_bounds__(256) void small_gemm_kernel(GemmProblem<T> p) {
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int r = wm + 16 * i + lane % 16;
-        a[i] = As[akc ? lds_at<true>(r, kk) : lds_at<false>(r, kk)];
+        a[i] = As[akc ? lds_at<true, KC>(r, kk) : lds_at<false, KC>(r, kk)];
       }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int r = wn + 16 * j + lane % 16;
-        bv[j] = Bs[bkc ? lds_at<true>(r, kk) : lds_at<false>(r, kk)];
+        bv[j] = Bs[bkc ? lds_at<true, KC>(r, kk) : lds_at<false, KC>(r, kk)];
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -224,8 +232,18 @@ inline bool eligible(pb_handle_t h, int64_t M, int64_t N, int64_t batch, bool in
 template <typename T>
 pb_status_t launch(pb_handle_t h, const GemmProblem<T>& p, bool vec) {
   const dim3 grid{unsigned(p.batch)};
-  if (vec) hipLaunchKernelGGL((small_gemm_kernel<T, true>), grid, dim3(256), 0, h->stream, p);
-  else hipLaunchKernelGGL((small_gemm_kernel<T, false>), grid, dim3(256), 0, h->stream, p);
+  if constexpr (sizeof(T) == 4) {
+    if (vec && p.M == kTile && p.N == kTile && p.K % 64 == 0) {
+      hipLaunchKernelGGL((small_gemm_kernel<T, true, 64, true>), grid, dim3(256), 0, h->stream, p);
+      PB_CHECK_LAUNCH();
+      return PB_STATUS_SUCCESS;
+    }
+  }
+  if (vec) {
+    hipLaunchKernelGGL((small_gemm_kernel<T, true, 32>), grid, dim3(256), 0, h->stream, p);
+  } else {
+    hipLaunchKernelGGL((small_gemm_kernel<T, false, 32>), grid, dim3(256), 0, h->stream, p);
+  }
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }

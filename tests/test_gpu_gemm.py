@@ -274,10 +274,11 @@ def test_strided_batched_4096_x_64(handle, torch_cuda):
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("ta,tb", [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")])
-@pytest.mark.parametrize("m,n,k,lda_pad", [(64, 64, 64, 0), (37, 50, 200, 3), (8, 61, 5, 1)])
+@pytest.mark.parametrize("m,n,k,lda_pad", [(64, 64, 64, 0), (64, 64, 192, 0), (37, 50, 200, 3), (8, 61, 5, 1)])
 def test_many_small_problems(handle, torch_cuda, dtype, ta, tb, m, n, k, lda_pad):
     """Batches of >= one problem per CU with M, N <= 64 take the
-    one-problem-per-workgroup kernel: ragged M/N, K over several 64-chunks,
+    one-problem-per-workgroup kernel: full 64 x 64 tiles with K in one or
+    three chunks of 64, ragged M/N, K over several chunks,
     unaligned leading dimensions (scalar staging), alpha/beta; 300 problems,
     checked on a sample against the FP64 oracle."""
     import portblas_amd as pb
