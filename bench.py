@@ -100,7 +100,8 @@ def committed_traffic():
 def pmc_probe(torch, pb, h, which="bf16x3"):
     """A plain 256 MiB copy, whose FETCH_SIZE / WRITE_SIZE give the counters'
     scale, then the BF16x3 kernel alone on packed 8192^3 operands (or
-    DGEMM 4096^3), twice."""
+    DGEMM 3072^3, small enough that SQ_VALU_MFMA_BUSY_CYCLES, summed over
+    the 1024 SIMDs, stays below its 2^32 limit), twice."""
     src = torch.ones(1 << 26, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -108,7 +109,7 @@ def pmc_probe(torch, pb, h, which="bf16x3"):
     torch.cuda.synchronize()
     del src, dst
     if which == "dgemm":
-        D = 4096
+        D = 3072
         Ad = torch.rand(D * D, dtype=torch.float64, device="cuda")
         Bd = torch.rand(D * D, dtype=torch.float64, device="cuda")
         Cd = torch.zeros(D * D, dtype=torch.float64, device="cuda")

@@ -34,99 +34,114 @@ pb_status_t sgemm_alpha_zero(pb_handle_t h, int64_t M, int64_t N, float beta, fl
 // ---------------------------------------------------------------------------
 // Pre-pass kernels
 // ---------------------------------------------------------------------------
-// amax over the logical rows x K elements of op(X) (all batches); |x| bits are
-// monotone for non-negative floats, so an integer max is exact.  NaN skipped.
-__global__ void amax_kernel(const float* __restrict__ x, int64_t rs, int64_t ks, int64_t rows,
-                            int64_t K, int64_t stride, int64_t batch, unsigned int* out) {
+// amax pre-pass.  Every workgroup writes the largest |x| it saw to its own
+// slot of a partial array (no atomics, nothing to clear first); the scales
+// kernel folds the partials.  NaN never wins: fmaxf returns the other
+// operand.
+constexpr int kAmaxBlock = 256;
+constexpr int kAmaxMaxParts = 2048;  // partial slots per operand (grid cap)
+
+// Largest value over the workgroup, written by thread 0 to part[blockIdx.x].
+__device__ __forceinline__ void amax_block_store(float best, float* part) {
+  __shared__ float red[kAmaxBlock / 64];
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) best = fmaxf(best, __shfl_xor(best, off, 64));
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    float m = red[0];
+#pragma unroll
+    for (int w = 1; w < kAmaxBlock / 64; ++w) m = fmaxf(m, red[w]);
+    part[blockIdx.x] = m;
+  }
+}
+
+// amax over the logical rows x K elements of op(X) (all batches), any strides.
+__global__ __launch_bounds__(kAmaxBlock) void amax_kernel(const float* __restrict__ x, int64_t rs, int64_t ks,
+                                                          int64_t rows, int64_t K, int64_t stride, int64_t batch,
+                                                          float* part) {
   const int64_t per = rows * K;
   const int64_t total = per * batch;
-  unsigned int best = 0;
+  float best = 0.f;
   for (int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < total;
        i += int64_t(gridDim.x) * blockDim.x) {
     const int64_t b = i / per, e = i % per;
     const int64_t r = rs == 1 ? e % rows : e / K;
     const int64_t k = rs == 1 ? e / rows : e % K;
-    const float v = fabsf(x[b * stride + r * rs + k * ks]);
-    if (v == v) {
-      unsigned int u;
-      __builtin_memcpy(&u, &v, 4);
-      best = u > best ? u : best;
-    }
+    best = fmaxf(best, fabsf(x[b * stride + r * rs + k * ks]));
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned int o = __shfl_xor(best, off, 64);
-    best = o > best ? o : best;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, best);
+  amax_block_store(best, part);
 }
 
 // amax of a dense block of n floats (an operand stored without padding):
-// 16-byte non-temporal loads, 4 in flight per lane.  NaN never wins (fmaxf
-// returns the other operand); the bits of a non-negative float order like
-// the float, so the workgroup results combine with an unsigned atomicMax.
-__global__ __launch_bounds__(256) void amax_flat_kernel(const float* __restrict__ x, int64_t n,
-                                                        unsigned int* out) {
+// 16-byte non-temporal loads, 8 in flight per lane.
+__global__ __launch_bounds__(kAmaxBlock) void amax_flat_kernel(const float* __restrict__ x, int64_t n,
+                                                               float* part) {
   typedef float f4 __attribute__((ext_vector_type(4)));
+  constexpr int U = 8;
   const int64_t nvec = n / 4;
   const f4* xv = reinterpret_cast<const f4*>(x);
   float best = 0.f;
-  const int64_t step = int64_t(gridDim.x) * 256 * 4;
-  for (int64_t base = int64_t(blockIdx.x) * 256 * 4 + threadIdx.x; base < nvec; base += step) {
-    f4 v[4];
+  const int64_t step = int64_t(gridDim.x) * kAmaxBlock * U;
+  for (int64_t base = int64_t(blockIdx.x) * kAmaxBlock * U + threadIdx.x; base < nvec; base += step) {
+    f4 v[U];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) v[u] = base + u * 256 < nvec ? __builtin_nontemporal_load(xv + base + u * 256) : f4{0, 0, 0, 0};
+    for (int u = 0; u < U; ++u)
+      v[u] = base + u * kAmaxBlock < nvec ? __builtin_nontemporal_load(xv + base + u * kAmaxBlock) : f4{0, 0, 0, 0};
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int u = 0; u < U; ++u)
       best = fmaxf(best, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
   }
   if (blockIdx.x == 0 && threadIdx.x < n - nvec * 4) best = fmaxf(best, fabsf(x[nvec * 4 + threadIdx.x]));
-  unsigned int ub;
-  __builtin_memcpy(&ub, &best, 4);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned int o = __shfl_xor(ub, off, 64);
-    ub = o > ub ? o : ub;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, ub);
+  amax_block_store(best, part);
 }
 
 // amax over `nlines` lines (per batch) of `len` contiguous floats, line j of
 // batch b at x + b * stride + j * pitch: one workgroup per line at a time,
 // coalesced along the line, no per-element index arithmetic.
-__global__ __launch_bounds__(256) void amax_lines_kernel(const float* __restrict__ x, int64_t len,
-                                                         int64_t pitch, int64_t nlines, int64_t stride,
-                                                         int64_t batch, unsigned int* out) {
+__global__ __launch_bounds__(kAmaxBlock) void amax_lines_kernel(const float* __restrict__ x, int64_t len,
+                                                                int64_t pitch, int64_t nlines, int64_t stride,
+                                                                int64_t batch, float* part) {
   float best = 0.f;
   for (int64_t l = blockIdx.x; l < nlines * batch; l += gridDim.x) {
     const float* line = x + (l / nlines) * stride + (l % nlines) * pitch;
-    for (int64_t i = threadIdx.x; i < len; i += 256) best = fmaxf(best, fabsf(line[i]));
+    for (int64_t i = threadIdx.x; i < len; i += kAmaxBlock) best = fmaxf(best, fabsf(line[i]));
   }
-  unsigned int ub;
-  __builtin_memcpy(&ub, &best, 4);
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const unsigned int o = __shfl_xor(ub, off, 64);
-    ub = o > ub ? o : ub;
-  }
-  if ((threadIdx.x & 63) == 0) atomicMax(out, ub);
+  amax_block_store(best, part);
 }
 
-// scales[0] = 448 / amax_a, scales[1] = 448 / amax_b, scales[2] = the
-// epilogue factor amax_a * amax_b / 448^2 (1 for an all-zero operand).
-__global__ void fp8_scales_kernel(const unsigned int* amax_bits, float* scales) {
-  float a, b;
-  __builtin_memcpy(&a, &amax_bits[0], 4);
-  __builtin_memcpy(&b, &amax_bits[1], 4);
+// Folds the two operands' partial maxima (na and nb slots) and derives the
+// per-tensor scales: scales[0] = 448 / amax(A), scales[1] = 448 / amax(B)
+// (1 when the amax is 0 or not finite), scales[2] = 1 / (scales[0] scales[1]).
+__global__ __launch_bounds__(kAmaxBlock) void fp8_scales_kernel(const float* pa, int na, const float* pb, int nb,
+                                                                float* scales) {
+  __shared__ float red[2][kAmaxBlock / 64];
+  float a = 0.f, b = 0.f;
+  for (int i = threadIdx.x; i < na; i += kAmaxBlock) a = fmaxf(a, pa[i]);
+  for (int i = threadIdx.x; i < nb; i += kAmaxBlock) b = fmaxf(b, pb[i]);
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = fmaxf(a, __shfl_xor(a, off, 64));
+    b = fmaxf(b, __shfl_xor(b, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+#pragma unroll
+  for (int w = 1; w < kAmaxBlock / 64; ++w) {
+    red[0][0] = fmaxf(red[0][0], red[0][w]);
+    red[1][0] = fmaxf(red[1][0], red[1][w]);
+  }
+  a = red[0][0];
+  b = red[1][0];
   const float sa = (a > 0.f && a < 3.0e38f) ? 448.0f / a : 1.0f;
   const float sb = (b > 0.f && b < 3.0e38f) ? 448.0f / b : 1.0f;
   scales[0] = sa;
   scales[1] = sb;
   scales[2] = (1.0f / sa) * (1.0f / sb);
-}
-
-__global__ void fill_zero_u32(unsigned int* p, int n) {
-  if (int(threadIdx.x) < n) p[threadIdx.x] = 0u;
 }
 
 enum PackMode { kPackBf16 = 0, kPackBf16Split = 1, kPackFp8 = 2 };
@@ -433,38 +448,43 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   void* b_hi = sc.take<uint8_t>(bbytes);
   void* b_lo = nparts == 2 ? sc.take<uint8_t>(bbytes) : nullptr;
   float* scales = sc.take<float>(4);
-  unsigned int* amax_bits = sc.take<unsigned int>(2);
+  float* amax_part = sc.take<float>(2 * kAmaxMaxParts);
   float* partial = part_bytes ? sc.take<float>(part_bytes / sizeof(float)) : nullptr;
 
   // Operand strides of op(A) (rows M) and op(B) (rows N), as for the engine.
   const int64_t a_rs = g.ta == 'n' ? 1 : g.lda, a_ks = g.ta == 'n' ? g.lda : 1;
   const int64_t b_rs = g.tb == 'n' ? g.ldb : 1, b_ks = g.tb == 'n' ? 1 : g.ldb;
   if (fp8) {
-    // amax of each operand over all batches, then the scales (device side).
-    hipLaunchKernelGGL(fill_zero_u32, dim3(1), dim3(64), 0, h->stream, amax_bits, 2);
-    // Dense operands stream as one block; others go line by line along their
-    // contiguous dimension.
-    auto amax = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, unsigned int* out) {
+    // amax of each operand over all batches (one partial per workgroup),
+    // then the scales (device side).  Dense operands stream as one block;
+    // others go line by line along their contiguous dimension.
+    auto amax = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, float* part) -> int {
       const int64_t per = rows * K;
       const bool dense = ((rs == 1 && ks == rows) || (ks == 1 && rs == K)) && (batch == 1 || stride == per);
+      int64_t grid;
       if (dense && aligned16(x)) {
-        const int64_t gv = std::min<int64_t>(pb::ceil_div(per * batch, 4096), 8 * h->num_cus);
-        hipLaunchKernelGGL(amax_flat_kernel, dim3(unsigned(std::max<int64_t>(gv, 1))), dim3(256), 0, h->stream, x,
-                           per * batch, out);
+        grid = std::min<int64_t>(pb::ceil_div(per * batch, 4 * 8 * kAmaxBlock), 8 * h->num_cus);
+        grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
+        hipLaunchKernelGGL(amax_flat_kernel, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x,
+                           per * batch, part);
       } else if (rs == 1 || ks == 1) {
         const int64_t len = rs == 1 ? rows : K, nl = rs == 1 ? K : rows, pitch = rs == 1 ? ks : rs;
-        const int64_t gl = std::min<int64_t>(nl * batch, 8 * h->num_cus);
-        hipLaunchKernelGGL(amax_lines_kernel, dim3(unsigned(std::max<int64_t>(gl, 1))), dim3(256), 0, h->stream,
-                           x, len, pitch, nl, stride, batch, out);
+        grid = std::min<int64_t>(nl * batch, 8 * h->num_cus);
+        grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
+        hipLaunchKernelGGL(amax_lines_kernel, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream,
+                           x, len, pitch, nl, stride, batch, part);
       } else {
-        const int64_t ge = std::min<int64_t>(pb::ceil_div(per * batch, 256), 4 * h->num_cus);
-        hipLaunchKernelGGL(amax_kernel, dim3(unsigned(std::max<int64_t>(ge, 1))), dim3(256), 0, h->stream, x, rs,
-                           ks, rows, K, stride, batch, out);
+        grid = std::min<int64_t>(pb::ceil_div(per * batch, kAmaxBlock), 4 * h->num_cus);
+        grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
+        hipLaunchKernelGGL(amax_kernel, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x, rs,
+                           ks, rows, K, stride, batch, part);
       }
+      return int(grid);
     };
-    amax(g.a, a_rs, a_ks, M, g.sa, amax_bits);
-    amax(g.b, b_rs, b_ks, N, g.sb, amax_bits + 1);
-    hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(1), 0, h->stream, amax_bits, scales);
+    const int na = amax(g.a, a_rs, a_ks, M, g.sa, amax_part);
+    const int nb = amax(g.b, b_rs, b_ks, N, g.sb, amax_part + kAmaxMaxParts);
+    hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
+                       amax_part + kAmaxMaxParts, nb, scales);
     PB_CHECK_LAUNCH();
   }
   PB_REQUIRE(batch <= 65535, "batch too large for the pre-pass");
