@@ -86,24 +86,31 @@ __global__ __launch_bounds__(256) void grouped_kernel(const DevProblem* __restri
   }
 }
 
-// BF16 problems: a lean 64 x 64 tile.  18 KB of LDS and about 120 registers
-// let four workgroups share a CU, so the latency of one tile's loads is
-// hidden both by the other resident tiles and by issuing the next chunk's
-// loads before the current chunk's MFMAs (most grouped problems are a
-// handful of tiles and a few K chunks).  Per K chunk
-// of 64 every thread issues its 2 + 2 16-byte loads, then writes them to LDS
-// K-contiguous ([row][k], pitch 72): a K-contiguous operand with one
-// 16-byte store, an M/N-contiguous one transposed with eight 2-byte stores.
-// Each wave computes a 32 x 32 quarter with v_mfma_f32_32x32x16_bf16.
-constexpr int kLT = 64, kLP = 72;
+// BF16 problems: a lean 64 x 64 tile.  10 KB of LDS and at most 96
+// registers let five workgroups share a CU, so the latency of one tile's
+// loads is hidden both by the other resident tiles and by issuing the next
+// chunk's loads before the current chunk's MFMAs (most grouped problems are a
+// handful of tiles and a few K chunks; more resident tiles beat longer
+// chunks: K chunks of 64 at four workgroups per CU, or of 128 at two or
+// three, were slower).  Per K chunk of 32 every thread issues one 16-byte
+// load per operand, then writes it to LDS K-contiguous ([row][k], pitch
+// 40): a K-contiguous operand with one 16-byte store, an M/N-contiguous one
+// transposed with eight 2-byte stores.  Each wave computes a 32 x 32 quarter
+// with v_mfma_f32_32x32x16_bf16.
+constexpr int kLT = 64;          // tile edge (M and N)
+constexpr int kLK = 32;          // K chunk staged per step
+constexpr int kLP = kLK + 8;     // LDS row pitch ([row][k], elements)
+constexpr int kLG = kLK / 8;     // 16-byte groups per K-contiguous row of a chunk
+constexpr int kLIt = kLT * kLG / 256;  // 16-byte loads per thread and operand
 
 template <bool KC>
 __device__ __forceinline__ void lean_load(const uint16_t* x, int64_t rs, int64_t ks, int64_t rows,
                                           int64_t K, int64_t r0, int64_t k0, bool vec, int e,
                                           u32x4& out) {
-  // e in [0, 512): KC -> row e / 8, k group (e % 8) * 8; else k e / 8, row group (e % 8) * 8.
-  const int64_t r = KC ? r0 + e / 8 : r0 + (e % 8) * 8;
-  const int64_t k = KC ? k0 + (e % 8) * 8 : k0 + e / 8;
+  // e in [0, kLT * kLG): KC -> row e / kLG, k group (e % kLG) * 8; else k e / 8,
+  // row group (e % 8) * 8.
+  const int64_t r = KC ? r0 + e / kLG : r0 + (e % 8) * 8;
+  const int64_t k = KC ? k0 + (e % kLG) * 8 : k0 + e / 8;
   if (vec && (KC ? (r < rows && k + 8 <= K) : (k < K && r + 8 <= rows))) {
     out = *reinterpret_cast<const u32x4*>(x + r * rs + k * ks);
     return;
@@ -123,7 +130,7 @@ __device__ __forceinline__ void lean_load(const uint16_t* x, int64_t rs, int64_t
 template <bool KC>
 __device__ __forceinline__ void lean_store(uint16_t (*dst)[kLP], int e, const u32x4& v) {
   if constexpr (KC) {
-    *reinterpret_cast<u32x4*>(&dst[e / 8][(e % 8) * 8]) = v;
+    *reinterpret_cast<u32x4*>(&dst[e / kLG][(e % kLG) * 8]) = v;
   } else {
     uint16_t h[8];
     __builtin_memcpy(h, &v, 16);
@@ -148,9 +155,9 @@ __device__ __forceinline__ LeanSrc lean_src(const DevProblem& P) {
 
 template <bool KCA, bool KCB>
 __device__ __forceinline__ void lean_chunk_load(const LeanSrc& S, int64_t m0, int64_t n0, int64_t k0,
-                                                u32x4 (&va)[2], u32x4 (&vb)[2]) {
+                                                u32x4 (&va)[kLIt], u32x4 (&vb)[kLIt]) {
 #pragma unroll
-  for (int it = 0; it < 2; ++it) {
+  for (int it = 0; it < kLIt; ++it) {
     const int e = threadIdx.x + 256 * it;
     lean_load<KCA>(S.a, S.a_rs, S.a_ks, S.M, S.K, m0, k0, S.vec, e, va[it]);
     lean_load<KCB>(S.b, S.b_rs, S.b_ks, S.N, S.K, n0, k0, S.vec, e, vb[it]);
@@ -172,7 +179,7 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   int4 cur = tiles[begin];
-  u32x4 va[2], vb[2];
+  u32x4 va[kLIt], vb[kLIt];
   lean_chunk_load<KCA, KCB>(lean_src(probs[cur.x]), int64_t(cur.y) * kLT, int64_t(cur.z) * kLT, 0, va, vb);
   for (int i = begin; i < end; ++i) {
     const DevProblem& P = probs[cur.x];
@@ -181,25 +188,25 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
     const int4 nxt = has_next ? tiles[i + 1] : cur;
     const LeanSrc Sn = lean_src(probs[nxt.x]);
     const int64_t m0 = int64_t(cur.y) * kLT, n0 = int64_t(cur.z) * kLT;
-    const int64_t kchunks = S.K > 0 ? (S.K + kLT - 1) / kLT : 1;
+    const int64_t kchunks = S.K > 0 ? (S.K + kLK - 1) / kLK : 1;
     f32x16 acc;
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.f;
     for (int64_t kc = 0; kc < kchunks; ++kc) {
       __syncthreads();  // the previous chunk (or tile) is consumed
 #pragma unroll
-      for (int it = 0; it < 2; ++it) {
+      for (int it = 0; it < kLIt; ++it) {
         const int e = threadIdx.x + 256 * it;
         lean_store<KCA>(As, e, va[it]);
         lean_store<KCB>(Bs, e, vb[it]);
       }
       __syncthreads();
       if (kc + 1 < kchunks)
-        lean_chunk_load<KCA, KCB>(S, m0, n0, (kc + 1) * kLT, va, vb);
+        lean_chunk_load<KCA, KCB>(S, m0, n0, (kc + 1) * kLK, va, vb);
       else if (has_next)
         lean_chunk_load<KCA, KCB>(Sn, int64_t(nxt.y) * kLT, int64_t(nxt.z) * kLT, 0, va, vb);
 #pragma unroll
-      for (int kk = 0; kk < kLT; kk += 16) {
+      for (int kk = 0; kk < kLK; kk += 16) {
         bf16x8 a, b;
         __builtin_memcpy(&a, &As[wm + lane % 32][kk + 8 * (lane / 32)], 16);
         __builtin_memcpy(&b, &Bs[wn + lane % 32][kk + 8 * (lane / 32)], 16);
@@ -290,7 +297,7 @@ void balance_tiles(const std::vector<DevProblem>& v, int grid, std::vector<int4>
   std::vector<Item> probs_by_cost;
   int64_t total = 0;
   for (size_t i = 0; i < v.size(); ++i) {
-    probs_by_cost.push_back({pb::ceil_div(std::max<int64_t>(v[i].K, 1), kLT) + 1, int(i)});
+    probs_by_cost.push_back({pb::ceil_div(std::max<int64_t>(v[i].K, 1), kLK) + 1, int(i)});
     total += probs_by_cost.back().cost * int64_t(v[i].tiles_m) * v[i].tiles_n;
   }
   std::stable_sort(probs_by_cost.begin(), probs_by_cost.end(), [&](const Item& x, const Item& y) {
