@@ -122,9 +122,10 @@ struct Chunk {
   }
 };
 
-// FULL: M == N == 64 and K a multiple of KC (VEC only).
+// FULL: M == N == 64 and K a multiple of KC (VEC only).  Workgroup b runs
+// problem b.
 template <typename T, bool VEC, int KC, bool FULL = false>
-__global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : 1) void small_gemm_kernel(GemmProblem<T> p) {
+__global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : (VEC && sizeof(T) == 4) ? 5 : 1) void small_gemm_kernel(GemmProblem<T> p) {
   using Acc = typename Traits<T>::AccV;  // 4 accumulators per lane (f32x4 / f64x4)
   constexpr int kKc = KC;
   __shared__ __attribute__((aligned(16))) T As[ChunkShape<KC>::kLdsElems];
@@ -132,12 +133,18 @@ __global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : 1) void small_gemm_ke
   // Layout per operand: [r][k] when K-contiguous and vector-staged.
   const bool akc = VEC && p.a_rs != 1 && p.a_ks == 1;
   const bool bkc = VEC && p.b_rs != 1 && p.b_ks == 1;
-  const int64_t b = blockIdx.x;
-  const T* A = p.a[0] + b * p.a_stride;
-  const T* B = p.b[0] + b * p.b_stride;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
   const int M = int(p.M), N = int(p.N);
+  const T alpha = p.alpha, beta = p.beta;
+  const int64_t b = blockIdx.x;
+  const T* A = p.a[0] + b * p.a_stride;
+  const T* B = p.b[0] + b * p.b_stride;
+  Chunk<T, VEC, KC, FULL> ca, cb;
+  if (p.K > 0) {
+    ca.load(A, p.a_rs, p.a_ks, M, p.K, 0);
+    cb.load(B, p.b_rs, p.b_ks, N, p.K, 0);
+  }
   Acc acc[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -145,13 +152,8 @@ __global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : 1) void small_gemm_ke
     for (int j = 0; j < 2; ++j)
 #pragma unroll
       for (int e = 0; e < 4; ++e) acc[i][j][e] = T(0);
-  Chunk<T, VEC, KC, FULL> ca, cb;
-  if (p.K > 0) {
-    ca.load(A, p.a_rs, p.a_ks, M, p.K, 0);
-    cb.load(B, p.b_rs, p.b_ks, N, p.K, 0);
-  }
   for (int64_t k0 = 0; k0 < p.K; k0 += kKc) {
-    if (k0 > 0) __syncthreads();  // previous chunk fully consumed
+    __syncthreads();  // the previous chunk is fully consumed
     ca.store(p.a_rs, akc, As);
     cb.store(p.b_rs, bkc, Bs);
     __syncthreads();
@@ -187,7 +189,6 @@ __global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : 1) void small_gemm_ke
   // C = alpha * acc + beta * C (beta == 0: C not read).  Accumulator e of a
   // lane holds column lane % 16 and row acc_row(lane, e).
   T* C = p.c + b * p.c_stride;
-  const T alpha = p.alpha, beta = p.beta;
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
