@@ -127,15 +127,27 @@ __device__ __forceinline__ void lean_load(const uint16_t* x, int64_t rs, int64_t
   out = u32x4{w[0], w[1], w[2], w[3]};
 }
 
+// 16-byte chunk c (8 k values) of LDS row r sits at chunk c ^ lean_swz(r).
+// The MFMA fragment reads (16 consecutive rows, one chunk) stay conflict-free
+// (one swizzle value per 16-row block), and the transposing 2-byte stores of
+// an M/N-contiguous operand (8 rows 8 apart, one chunk) spread over 8 bank
+// groups instead of 2.
+__device__ __forceinline__ int lean_swz(int r) { return (r >> 4) & (kLG - 1); }
+
 template <bool KC>
 __device__ __forceinline__ void lean_store(uint16_t (*dst)[kLP], int e, const u32x4& v) {
   if constexpr (KC) {
-    *reinterpret_cast<u32x4*>(&dst[e / kLG][(e % kLG) * 8]) = v;
+    const int r = e / kLG;
+    *reinterpret_cast<u32x4*>(&dst[r][((e % kLG) ^ lean_swz(r)) * 8]) = v;
   } else {
     uint16_t h[8];
     __builtin_memcpy(h, &v, 16);
+    const int k = e / 8;
 #pragma unroll
-    for (int i = 0; i < 8; ++i) dst[(e % 8) * 8 + i][e / 8] = h[i];
+    for (int i = 0; i < 8; ++i) {
+      const int r = (e % 8) * 8 + i;
+      dst[r][((k / 8) ^ lean_swz(r)) * 8 + k % 8] = h[i];
+    }
   }
 }
 
@@ -208,8 +220,9 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
 #pragma unroll
       for (int kk = 0; kk < kLK; kk += 16) {
         bf16x8 a, b;
-        __builtin_memcpy(&a, &As[wm + lane % 32][kk + 8 * (lane / 32)], 16);
-        __builtin_memcpy(&b, &Bs[wn + lane % 32][kk + 8 * (lane / 32)], 16);
+        const int ra = wm + lane % 32, rb = wn + lane % 32, c = kk / 8 + lane / 32;
+        __builtin_memcpy(&a, &As[ra][(c ^ lean_swz(ra)) * 8], 16);
+        __builtin_memcpy(&b, &Bs[rb][(c ^ lean_swz(rb)) * 8], 16);
         acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
       }
     }
