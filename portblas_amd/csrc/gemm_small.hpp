@@ -7,11 +7,13 @@
 // 32 x 32 quarter of C with v_mfma_f32_16x16x4_f32 (FP32) or
 // v_mfma_f64_16x16x4_f64 (FP64).  Full 64 x 64 FP32 problems with K a
 // multiple of 64 stage K in chunks of 64 without bounds checks (a 64^3
-// problem is one memory latency); the others use chunks of 32.  The next
-// chunk's loads are issued before the current chunk's MFMAs.  An operand keeps its contiguous dimension in LDS:
-// row-contiguous as [k][r] (pitch 80 elements, 16-byte stores), K-contiguous
-// as [r][k] (pitch KC + 4, 16-byte stores); with 64 LDS banks both make the
-// 16-lane x 4-k MFMA fragment reads conflict-free.
+// problem is one memory latency; four workgroups per CU); the others use
+// chunks of 32 (five vector-staged FP32 workgroups per CU).  The next
+// chunk's loads are issued before the current chunk's MFMAs.  An operand
+// keeps its contiguous dimension in LDS: row-contiguous as [k][r] (pitch 80
+// elements, 16-byte stores), K-contiguous as [r][k] (pitch KC + 4, 16-byte
+// stores); with 64 LDS banks both make the 16-lane x 4-k MFMA fragment reads
+// conflict-free.
 #pragma once
 
 #include "gemm_launch.hpp"
@@ -125,7 +127,8 @@ struct Chunk {
 // FULL: M == N == 64 and K a multiple of KC (VEC only).  Workgroup b runs
 // problem b.
 template <typename T, bool VEC, int KC, bool FULL = false>
-__global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : (VEC && sizeof(T) == 4) ? 5 : 1) void small_gemm_kernel(GemmProblem<T> p) {
+__global__ __launch_bounds__(256, (KC == 64 && FULL) ? 4 : (VEC && sizeof(T) == 4) ? 5 : 1)
+void small_gemm_kernel(GemmProblem<T> p) {
   using Acc = typename Traits<T>::AccV;  // 4 accumulators per lane (f32x4 / f64x4)
   constexpr int kKc = KC;
   __shared__ __attribute__((aligned(16))) T As[ChunkShape<KC>::kLdsElems];
