@@ -126,6 +126,37 @@ def test_lowp_wide_range_inputs(handle, torch_cuda, dist):
         check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], "n", "n", 4096, 4096, 512, dist=dist, seed=5)
 
 
+@pytest.mark.parametrize("pos", ["first", "last"])
+def test_fp8_amax_over_every_workgroup(handle, torch_cuda, pos):
+    """The per-tensor amax folds one partial per pre-pass workgroup: a needle
+    (1000, against U[-1,1] elsewhere) at the first or last element of a
+    16.8M-element operand, which the first or last of the 2048 workgroups
+    sees, must set the scale (a missed partial would saturate the needle at
+    448 / amax(rest) and break the error bound)."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    m = n = 64
+    k = 1 << 18
+    idx = 0 if pos == "first" else -1
+    A = make(torch, m * k, 31)
+    B = make(torch, k * n, 32)
+    A[idx] = 1000.0
+    B[idx] = -1000.0
+    C = torch.zeros(m * n, device="cuda")
+    pb._gemm_ex(handle, pb.gemm_precision_t.fp8_e4m3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    torch.cuda.synchronize()
+    oa, ob = cm(A, m, k, m).double(), cm(B, k, n, k).double()
+    ref = oa @ ob
+    absprod = oa.abs() @ ob.abs()
+    ea = eb = 2.0 ** -10 * 1000.0 / 448
+    tol = (0.13 + 1.01 * (k + 2) * U) * absprod + 1.07 * (ea * ob.abs().sum(0, keepdim=True)
+                                                       + eb * oa.abs().sum(1, keepdim=True)) + k * ea * eb
+    err = (cm(C, m, n, m).double() - ref).abs()
+    assert torch.isfinite(err).all()
+    assert (err <= tol).all(), (err / tol).max().item()
+
+
 def test_bf16x4_not_less_accurate_than_bf16x3(handle, torch_cuda):
     """The extra lo*lo term can only shrink the representation error: on
     normal inputs the mean componentwise error must drop."""
