@@ -126,6 +126,42 @@ def test_lowp_wide_range_inputs(handle, torch_cuda, dist):
         check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], "n", "n", 4096, 4096, 512, dist=dist, seed=5)
 
 
+@pytest.mark.parametrize("prec", PRECS)
+@pytest.mark.parametrize("ta,tb,m,n,k,batch", [
+    ("n", "n", 4096, 4096, 1000, 1),  # K ends inside the last packed K step
+    ("t", "n", 4000, 4100, 2040, 1),
+    ("n", "t", 4096, 4352, 3200, 2),  # K a whole number of ring revolutions, batched
+])
+def test_lowp_exact_on_integer_inputs(handle, torch_cuda, prec, ta, tb, m, n, k, batch):
+    """Integers in [-2, 2] are exact in every format (BF16 hi with lo = 0;
+    FP8 after the per-tensor scale 448 / 2 = 224, which maps them to 0, 224,
+    448), their products and FP32 sums are exact, so every K step must land
+    exactly once: the result is the integer product up to the final
+    dequantising multiply (a step dropped, repeated or read from the wrong
+    ring stage moves it by whole integers)."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    g = torch.Generator(device="cuda").manual_seed(m + n + k)
+    ar, ac = (m, k) if ta == "n" else (k, m)
+    br, bc = (k, n) if tb == "n" else (n, k)
+    A = torch.randint(-2, 3, (ar * ac * batch,), generator=g, device="cuda").float()
+    B = torch.randint(-2, 3, (br * bc * batch,), generator=g, device="cuda").float()
+    C = torch.full((m * n * batch,), float("nan"), device="cuda")
+    pb._gemm_ex(handle, pb.gemm_precision_t[prec], ta, tb, m, n, k, 1.0, A, ar, B, br, 0.0, C, m,
+                ar * ac, br * bc, m * n, batch)
+    torch.cuda.synchronize()
+    for b in range(batch):
+        oa = cm(A[b * ar * ac:], ar, ac, ar).double()
+        ob = cm(B[b * br * bc:], br, bc, br).double()
+        oa = oa if ta == "n" else oa.t()
+        ob = ob if tb == "n" else ob.t()
+        ref = oa @ ob
+        got = cm(C[b * m * n:], m, n, m).double()
+        assert torch.isfinite(got).all()
+        assert ((got - ref).abs() <= 2.0 ** -20 * ref.abs()).all(), (b, (got - ref).abs().max().item())
+
+
 @pytest.mark.parametrize("pos", ["first", "last"])
 def test_fp8_amax_over_every_workgroup(handle, torch_cuda, pos):
     """The per-tensor amax folds one partial per pre-pass workgroup: a needle
