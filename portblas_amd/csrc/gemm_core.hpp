@@ -206,6 +206,11 @@ struct Tile {
   static constexpr int WM = Shape::kWM, WN = Shape::kWN;
   static constexpr int FM = WM / MR, FN = WN / MR;
   static constexpr int LDK = BK + VW;  // LDS row pitch (elements), 16-byte pad
+  // FP64 64 x 64 tiles keep two K tiles in flight (a second register stage):
+  // at two workgroups per CU their MFMA work per K tile is too short to
+  // cover one load latency.  (For FP32 the extra registers cost a third
+  // resident workgroup and measured slower.)
+  static constexpr bool kDeep = BM * BN <= 64 * 64 && sizeof(Raw) == 8;
   static constexpr int A_ELEMS = BM * LDK, B_ELEMS = BN * LDK;
   static constexpr int STAGE_ELEMS = A_ELEMS + B_ELEMS;
   static constexpr int LDS_BYTES = 2 * STAGE_ELEMS * int(sizeof(Raw));
@@ -440,23 +445,57 @@ struct Tile {
           pb[t] = B.base + (n0 + rb * VW) * B.rs + (k_begin + kb * VW) * B.ks;
         }
         const int64_t step_a = int64_t(BK) * A.ks, step_b = int64_t(BK) * B.ks;
-        load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, st);
-        lds_store(st, lds);
-        __syncthreads();
-        for (int64_t kt = 0; kt < nk; ++kt) {
-          Raw* cur = lds + (kt & 1) * STAGE_ELEMS;
-          Raw* nxt = lds + ((kt + 1) & 1) * STAGE_ELEMS;
-          const bool more = kt + 1 < nk;
-          if (more) {
+        auto advance = [&]() {
 #pragma unroll
-            for (int t = 0; t < APT; ++t) pa[t] += step_a;
+          for (int t = 0; t < APT; ++t) pa[t] += step_a;
 #pragma unroll
-            for (int t = 0; t < BPTB; ++t) pb[t] += step_b;
-            load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, st);
+          for (int t = 0; t < BPTB; ++t) pb[t] += step_b;
+        };
+        if constexpr (kDeep) {
+          // Two K tiles in flight: tile kt + 2 is loaded while tile kt is
+          // consumed and tile kt + 1 waits in the other register stage.
+          Stage st2;
+          load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, st);
+          if (nk > 1) {
+            advance();
+            load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, st2);
           }
-          consume(cur);
-          if (more) lds_store(st, nxt);
+          lds_store(st, lds);
           __syncthreads();
+          // hold: tile kt + 1 (in flight); spare: free for tile kt + 2.
+          auto step = [&](int64_t kt, Raw* cur, Raw* nxt, Stage& hold, Stage& spare) {
+            if (kt + 2 < nk) {
+              advance();
+              load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, spare);
+            }
+            consume(cur);
+            if (kt + 1 < nk) lds_store(hold, nxt);
+            __syncthreads();
+          };
+          Raw* lds1 = lds + STAGE_ELEMS;
+          for (int64_t kt = 0; kt < nk; kt += 2) {
+            step(kt, lds, lds1, st2, st);
+            if (kt + 1 < nk) step(kt + 1, lds1, lds, st, st2);
+          }
+        } else {
+          load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, st);
+          lds_store(st, lds);
+          __syncthreads();
+          for (int64_t kt = 0; kt < nk; ++kt) {
+            Raw* cur = lds + (kt & 1) * STAGE_ELEMS;
+            Raw* nxt = lds + ((kt + 1) & 1) * STAGE_ELEMS;
+            const bool more = kt + 1 < nk;
+            if (more) {
+#pragma unroll
+              for (int t = 0; t < APT; ++t) pa[t] += step_a;
+#pragma unroll
+              for (int t = 0; t < BPTB; ++t) pb[t] += step_b;
+              load_all(pa, pb, A.rs, A.ks, B.rs, B.ks, st);
+            }
+            consume(cur);
+            if (more) lds_store(st, nxt);
+            __syncthreads();
+          }
         }
       }
       const int64_t kt0 = k_begin + nk * BK;
