@@ -51,7 +51,7 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--quick", action="store_true", help="headline + roofline only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--pmc-probe", nargs="?", const="bf16x3", choices=("bf16x3", "dgemm"), default=None,
+    ap.add_argument("--pmc-probe", nargs="?", const="bf16x3", choices=("bf16x3", "dgemm", "grouped"), default=None,
                     help="for rocprofv3 --pmc: one 256 MiB device copy (the counter scale), then two "
                          "launches of the BF16x3 kernel on packed 8192^3 operands (or of DGEMM 4096^3); "
                          "prints no JSON")
@@ -101,7 +101,8 @@ def pmc_probe(torch, pb, h, which="bf16x3"):
     """A plain 256 MiB copy, whose FETCH_SIZE / WRITE_SIZE give the counters'
     scale, then the BF16x3 kernel alone on packed 8192^3 operands (or
     DGEMM 3072^3, small enough that SQ_VALU_MFMA_BUSY_CYCLES, summed over
-    the 1024 SIMDs, stays below its 2^32 limit), twice."""
+    the 1024 SIMDs, stays below its 2^32 limit; or the 512-problem grouped
+    set), twice."""
     src = torch.ones(1 << 26, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -116,6 +117,15 @@ def pmc_probe(torch, pb, h, which="bf16x3"):
         for _ in range(2):
             pb._gemm(h, "n", "n", D, D, D, 1.0, Ad, D, Bd, D, 0.0, Cd, D)
             torch.cuda.synchronize()
+        log("pmc probe done")
+        return
+    if which == "grouped":
+        class Once:  # bench_grouped's timer interface: one call, then wait
+            def __call__(self, fn, reps, warm=1, before=None):
+                fn()
+                torch.cuda.synchronize()
+                return 1.0
+        bench_grouped(torch, pb, h, Once())
         log("pmc probe done")
         return
     L = pb.load()
