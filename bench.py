@@ -4,7 +4,7 @@ Headline (the JSON `value`): FP32-accurate SGEMM at M = N = K = 8192 through
 BF16x3 split precision on MFMA (pb_gemm_ex, PB_GEMM_BF16X3), in FP32-equivalent
 TFLOP/s (2 M N K per step).  One step = one full call: the split/pack pre-pass
 and the 3-term BF16 MFMA GEMM, inputs resident in HBM.  The same JSON line
-carries the rest of the BASELINE metric and configs:
+carries the rest of the headline metric and its configurations:
 
 * the plain FP32 MFMA SGEMM on the same inputs and the BF16x3 speedup;
 * the precision study: max / mean componentwise and normwise error of FP32

@@ -1,7 +1,7 @@
 """Autotuning + learned-selector pipeline (GPU).
 
     python -m portblas_amd.autotune --train 20000 --heldout 2000 \\
-        --out portblas_amd/selector_mi355x.npz --report gpurun_out/tune_report.json
+        --out portblas_amd/selector_mi355x.npz --report tune_report.json
 
 1. draws training and held-out shapes (M, N, K log-uniform in [16, 16384],
    batch in {1..256}, random transposes; work and footprint capped so one

@@ -1,6 +1,6 @@
 """dot / asum / nrm2 / iamax / iamin on the GPU vs the oracle and the CBLAS
 golden fixtures.  Sweeps follow test/unittest/blas1/blas1_{dot,asum,nrm2}_test.cpp
-and blas1_iaminmax_common.hpp; full-size cases use BASELINE's n = 2^26."""
+and blas1_iaminmax_common.hpp; full-size cases use the benchmark's n = 2^26."""
 from __future__ import annotations
 
 import numpy as np

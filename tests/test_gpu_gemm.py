@@ -253,7 +253,7 @@ def test_gemm_batched_interleaved(handle, torch_cuda, ta, tb):
 
 
 def test_strided_batched_4096_x_64(handle, torch_cuda):
-    """BASELINE's strided set (4096 x 64^3, stride = matrix size) on 64
+    """The benchmark's strided set (4096 x 64^3, stride = matrix size) on 64
     sampled batches."""
     import portblas_amd as pb
 

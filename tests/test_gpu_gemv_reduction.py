@@ -104,7 +104,7 @@ def test_gemv_t_row_chunks(handle, torch_cuda):
 
 @pytest.mark.parametrize("tr", ["n", "t"])
 def test_gemv_full_size_rows(handle, torch_cuda, tr):
-    """BASELINE's 8192 x 8192 SGEMV, alpha 1 beta 0; 64 output entries are
+    """The benchmark's 8192 x 8192 SGEMV, alpha 1 beta 0; 64 output entries are
     checked against the FP64 oracle (the remaining ones by linearity)."""
     import portblas_amd as pb
 

@@ -64,7 +64,7 @@ def test_selector_within_5pct_of_exhaustive_on_heldout_shapes():
 
 
 def test_exhaustive_fixture_is_consistent():
-    """Exhaustive search is the judge: its timings are positive and finite
+    """Exhaustive search is the yardstick: its timings are positive and finite
     for every configuration, and the features encode the recorded shapes."""
     from portblas_amd import tuning
 
