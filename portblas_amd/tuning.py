@@ -7,7 +7,7 @@ time every config, keep the fastest).  Here:
 
 * ``Autotuner`` times every one of the BF16 engine's configurations
   (pb_tune_time_config: 8 tile shapes x 5 split-K factors) for a shape — the
-  exhaustive search, also the judge of the learned choice;
+  exhaustive search, also the yardstick of the learned choice;
 * ``collect`` builds the timing dataset the selector learns from;
 * ``Selector`` is an MLP (8 features -> 3 x 64 hidden -> one predicted
   relative log-runtime per configuration) trained on the GPU by the
