@@ -102,6 +102,18 @@ def test_gemv_t_row_chunks(handle, torch_cuda):
         _gemv_check(handle, torch_cuda, "t", m, 130, 1.25, 0.5, m, 1, 1, seed=m, dtype=dt)
 
 
+@pytest.mark.parametrize("m,n,lda,incx,incy,alpha,beta,poison", [
+    (8224, 600, 8224, 1, 1, 1.5, 0.5, False),    # a partial last group of tiles, beta reads y
+    (8192, 1000, 8200, 2, -1, -1.0, 0.0, True),  # padded lda, strided x, reversed y, y never read
+    (16384, 37, 16384, -3, 3, 1.0, 2.0, False),  # fewer columns than waves in a workgroup
+])
+def test_gemv_n_tall(handle, torch_cuda, m, n, lda, incx, incy, alpha, beta, poison):
+    """Tall FP32 'n' problems (32 row tiles per CU and more, few or many
+    columns): vs the FP64 oracle with increments, padding, alpha/beta and the
+    beta == 0 no-read rule."""
+    _gemv_check(handle, torch_cuda, "n", m, n, alpha, beta, lda, incx, incy, seed=m + n, poison_y=poison)
+
+
 @pytest.mark.parametrize("tr", ["n", "t"])
 def test_gemv_full_size_rows(handle, torch_cuda, tr):
     """The benchmark's 8192 x 8192 SGEMV, alpha 1 beta 0; 64 output entries are
