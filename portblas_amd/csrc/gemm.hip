@@ -352,13 +352,20 @@ pb_status_t symm_entry(pb_handle_t h, char side, char uplo, int64_t M, int64_t N
 // and overwrites B with X.  The reference inverts 16 x 16 diagonal blocks
 // and walks them with two GEMMs per block.  Here the diagonal blocks are
 // 64 x 64 (all inverted by one launch, one workgroup per block, in LDS), so
-// a solve of order k takes ceil(k / 64) steps of two MFMA GEMMs:
+// a solve of order k takes ceil(k / 64) steps, each an MFMA GEMM
 //   X_i <- op(inv(A_ii)) (alpha_i W_i)
-//   W_rest <- alpha_i W_rest - op(A)_(rest, i) X_i
 // where W is a workspace copy of B, alpha_i is alpha at the first step and 1
 // after, and the steps run forward or backward as op(A) is lower or upper.
+// The steps are grouped in panels of kTrsmPanel blocks: after block i the
+// panel's later blocks get W_p <- alpha_i W_p - op(A)_(p, i) X_i, and after the
+// panel the rows (columns) beyond it get one update of rank up to 512 from
+// the whole panel, so W is read and written once per panel instead of once
+// per block.  Below order kTrsmPanelMinK the panel is one block: the wider
+// updates do not pay for the extra launches there.
 // ---------------------------------------------------------------------------
 constexpr int kTrsmNb = 64;
+constexpr int kTrsmPanel = 8;
+constexpr int64_t kTrsmPanelMinK = 2048;
 
 // Inverts the triangular diagonal block `blk` of A (order min(64, k - 64 blk),
 // padded to 64 with the identity) into inv + blk * 64 * 64 (ld 64, same
@@ -434,10 +441,15 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
   const int64_t nblk = pb::ceil_div(k, kTrsmNb);
   PB_REQUIRE(nblk <= 0x7fffffffLL, "matrix too large");
   // Workspace: [split-K partials of the largest GEMM step][inverted blocks][W].
+  // The steps multiply by one diagonal block (K = 64) or by a panel of
+  // kTrsmPanel blocks (K up to 512).
   GemmCall<T> big;
   big.ta = t; big.tb = 'n'; big.M = M; big.N = N; big.K = kTrsmNb;
   if (s == 'r') { big.ta = 'n'; big.tb = t; }
-  const size_t part = pb::align_up(plain_partial_bytes<T>(h, big), 256);
+  GemmCall<T> big_panel = big;
+  big_panel.K = std::min<int64_t>(k, int64_t(kTrsmNb) * kTrsmPanel);
+  const size_t part = pb::align_up(
+      std::max(plain_partial_bytes<T>(h, big), plain_partial_bytes<T>(h, big_panel)), 256);
   const size_t inv_bytes = pb::align_up(size_t(nblk) * kTrsmNb * kTrsmNb * sizeof(T), 256);
   void* ws = nullptr;
   PB_TRY(pb::workspace(h, part + inv_bytes + size_t(M) * size_t(N) * sizeof(T), &ws));
@@ -465,31 +477,43 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
   };
   // A(r, c) of op(A): A + r + c lda ('n') or, transposed, A + c + r lda.
   auto opA = [&](int64_t r, int64_t c) { return t == 'n' ? a + r + c * lda : a + c + r * lda; };
-  const int64_t last = (nblk - 1) * kTrsmNb;
   const bool forward = (s == 'l') == lower_eff;  // left+lower or right+upper walk from block 0
-  for (int64_t step = 0; step < nblk; ++step) {
-    const int64_t i = forward ? step * kTrsmNb : last - step * kTrsmNb;
-    const int64_t bs = std::min<int64_t>(kTrsmNb, k - i);
-    const T al = step == 0 ? alpha : T(1);
+  // Block q of the walk (0 first): rows (or columns) [start, start + size).
+  auto blk_start = [&](int64_t q) { return (forward ? q : nblk - 1 - q) * kTrsmNb; };
+  auto blk_size = [&](int64_t q) { return std::min<int64_t>(kTrsmNb, k - blk_start(q)); };
+  // X_i <- op(inv_ii) (al W_i)  (left)  or  (al W_i) op(inv_ii)  (right).
+  auto solve = [&](int64_t i, int64_t bs, T al) -> pb_status_t {
     const T* invb = inv + (i / kTrsmNb) * kTrsmNb * kTrsmNb;
-    if (s == 'l') {
-      // X_i (rows i..i+bs of B) <- op(inv_ii) (al W_i)
-      PB_TRY(gemm(t, 'n', bs, N, bs, al, invb, kTrsmNb, W + i, ldw, T(0), b + i, ldb));
-      if (forward) {  // rows below
-        PB_TRY(gemm(t, 'n', M - i - bs, N, bs, T(-1), opA(i + bs, i), lda, b + i, ldb, al, W + i + bs, ldw));
-      } else {  // rows above
-        PB_TRY(gemm(t, 'n', i, N, bs, T(-1), opA(0, i), lda, b + i, ldb, al, W, ldw));
-      }
-    } else {
-      // X_i (columns i..i+bs of B) <- (al W_i) op(inv_ii)
-      PB_TRY(gemm('n', t, M, bs, bs, al, W + i * ldw, ldw, invb, kTrsmNb, T(0), b + i * ldb, ldb));
-      if (forward) {  // columns to the right
-        PB_TRY(gemm('n', t, M, N - i - bs, bs, T(-1), b + i * ldb, ldb, opA(i, i + bs), lda, al,
-                    W + (i + bs) * ldw, ldw));
-      } else {  // columns to the left
-        PB_TRY(gemm('n', t, M, i, bs, T(-1), b + i * ldb, ldb, opA(i, 0), lda, al, W, ldw));
-      }
+    if (s == 'l') return gemm(t, 'n', bs, N, bs, al, invb, kTrsmNb, W + i, ldw, T(0), b + i, ldb);
+    return gemm('n', t, M, bs, bs, al, W + i * ldw, ldw, invb, kTrsmNb, T(0), b + i * ldb, ldb);
+  };
+  // W_[r0, r1) <- be W_[r0, r1) - op(A)[r0, r1) x [c0, c1) X_[c0, c1)  (left; the
+  // right side is the transposed statement on columns).
+  auto update = [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1, T be) -> pb_status_t {
+    if (r1 <= r0 || c1 <= c0) return PB_STATUS_SUCCESS;
+    if (s == 'l')
+      return gemm(t, 'n', r1 - r0, N, c1 - c0, T(-1), opA(r0, c0), lda, b + c0, ldb, be, W + r0, ldw);
+    return gemm('n', t, M, r1 - r0, c1 - c0, T(-1), b + c0 * ldb, ldb, opA(c0, r0), lda, be, W + r0 * ldw,
+                ldw);
+  };
+  // Inside a panel each solved block updates only the panel's later blocks;
+  // the rest of W gets one update per panel.  W is scaled by alpha where it
+  // is first touched.
+  const int64_t panel = k >= kTrsmPanelMinK ? kTrsmPanel : 1;
+  for (int64_t q0 = 0; q0 < nblk; q0 += panel) {
+    const int64_t q1 = std::min<int64_t>(q0 + panel, nblk);
+    const int64_t plo = forward ? blk_start(q0) : blk_start(q1 - 1);
+    const int64_t phi = forward ? blk_start(q1 - 1) + blk_size(q1 - 1) : blk_start(q0) + blk_size(q0);
+    for (int64_t q = q0; q < q1; ++q) {
+      const int64_t i = blk_start(q), bs = blk_size(q);
+      const T al = q == 0 ? alpha : T(1);
+      PB_TRY(solve(i, bs, al));
+      if (forward) PB_TRY(update(i + bs, phi, i, i + bs, al));
+      else PB_TRY(update(plo, i, i, i + bs, al));
     }
+    const T be = q0 == 0 ? alpha : T(1);
+    if (forward) PB_TRY(update(phi, k, plo, phi, be));
+    else PB_TRY(update(0, plo, plo, phi, be));
   }
   return PB_STATUS_SUCCESS;
 }

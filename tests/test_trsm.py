@@ -99,6 +99,17 @@ def test_gpu_dtrsm_and_unused_zero(handle, torch_cuda, side, uplo, trans, diag):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("side,uplo,trans,diag", COMBOS)
+def test_gpu_trsm_panels(handle, torch_cuda, side, uplo, trans, diag):
+    """Order 2100 (33 diagonal blocks, the last of 52): the solve walks panels
+    of several blocks, each followed by one wider trailing update."""
+    M, N = (2100, 40) if side == "l" else (40, 2100)
+    _check(handle, torch_cuda, np.float32, side, uplo, trans, diag, M, N, np.nan, seed=11)
+    if diag == "n":
+        _check(handle, torch_cuda, np.float64, side, uplo, trans, diag, M, N, 0.0, seed=12)
+
+
+@pytest.mark.gpu
 def test_gpu_trsm_argument_errors(handle, torch_cuda):
     import portblas_amd as pb
 
