@@ -440,16 +440,56 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
   const bool lower_eff = (u == 'l') == (t == 'n');  // op(A) is lower triangular
   const int64_t nblk = pb::ceil_div(k, kTrsmNb);
   PB_REQUIRE(nblk <= 0x7fffffffLL, "matrix too large");
-  // Workspace: [split-K partials of the largest GEMM step][inverted blocks][W].
-  // The steps multiply by one diagonal block (K = 64) or by a panel of
-  // kTrsmPanel blocks (K up to 512).
-  GemmCall<T> big;
-  big.ta = t; big.tb = 'n'; big.M = M; big.N = N; big.K = kTrsmNb;
-  if (s == 'r') { big.ta = 'n'; big.tb = t; }
-  GemmCall<T> big_panel = big;
-  big_panel.K = std::min<int64_t>(k, int64_t(kTrsmNb) * kTrsmPanel);
-  const size_t part = pb::align_up(
-      std::max(plain_partial_bytes<T>(h, big), plain_partial_bytes<T>(h, big_panel)), 256);
+  const bool forward = (s == 'l') == lower_eff;  // left+lower or right+upper walk from block 0
+  // Block q of the walk (0 first): rows (or columns) [start, start + size).
+  auto blk_start = [&](int64_t q) { return (forward ? q : nblk - 1 - q) * kTrsmNb; };
+  auto blk_size = [&](int64_t q) { return std::min<int64_t>(kTrsmNb, k - blk_start(q)); };
+  const int64_t panel = k >= kTrsmPanelMinK ? kTrsmPanel : 1;
+  // The walk, in launch order: solve(i, bs, al) solves block [i, i + bs);
+  // update(r0, r1, c0, c1, be) takes the solved blocks [c0, c1) out of the
+  // rows (columns) [r0, r1) of W.  Inside a panel each solved block updates
+  // only the panel's later blocks; the rest of W gets one update per panel.
+  // W is scaled by alpha where it is first touched.
+  auto walk = [&](auto&& solve, auto&& update) -> pb_status_t {
+    for (int64_t q0 = 0; q0 < nblk; q0 += panel) {
+      const int64_t q1 = std::min<int64_t>(q0 + panel, nblk);
+      const int64_t plo = forward ? blk_start(q0) : blk_start(q1 - 1);
+      const int64_t phi = forward ? blk_start(q1 - 1) + blk_size(q1 - 1) : blk_start(q0) + blk_size(q0);
+      for (int64_t q = q0; q < q1; ++q) {
+        const int64_t i = blk_start(q), bs = blk_size(q);
+        const T al = q == 0 ? alpha : T(1);
+        PB_TRY(solve(i, bs, al));
+        if (forward) PB_TRY(update(i + bs, phi, i, i + bs, al));
+        else PB_TRY(update(plo, i, i, i + bs, al));
+      }
+      const T be = q0 == 0 ? alpha : T(1);
+      if (forward) PB_TRY(update(phi, k, plo, phi, be));
+      else PB_TRY(update(0, plo, plo, phi, be));
+    }
+    return PB_STATUS_SUCCESS;
+  };
+  // Workspace: [split-K partials][inverted blocks][W].  The partial region
+  // holds the largest split of any GEMM of the walk: a narrow trailing update
+  // can split K where the full-width one does not, and must not fall back to
+  // the handle workspace, which holds the blocks and W.
+  size_t part = 0;
+  {
+    auto need = [&](int64_t m, int64_t n, int64_t kk) {
+      GemmCall<T> g;
+      g.M = m; g.N = n; g.K = kk;
+      if (m > 0 && n > 0) part = std::max(part, plain_partial_bytes<T>(h, g));
+    };
+    PB_TRY(walk(
+        [&](int64_t, int64_t bs, T) -> pb_status_t {
+          if (s == 'l') need(bs, N, bs); else need(M, bs, bs);
+          return PB_STATUS_SUCCESS;
+        },
+        [&](int64_t r0, int64_t r1, int64_t c0, int64_t c1, T) -> pb_status_t {
+          if (r1 > r0 && c1 > c0) { if (s == 'l') need(r1 - r0, N, c1 - c0); else need(M, r1 - r0, c1 - c0); }
+          return PB_STATUS_SUCCESS;
+        }));
+  }
+  part = pb::align_up(part, 256);
   const size_t inv_bytes = pb::align_up(size_t(nblk) * kTrsmNb * kTrsmNb * sizeof(T), 256);
   void* ws = nullptr;
   PB_TRY(pb::workspace(h, part + inv_bytes + size_t(M) * size_t(N) * sizeof(T), &ws));
@@ -477,10 +517,6 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
   };
   // A(r, c) of op(A): A + r + c lda ('n') or, transposed, A + c + r lda.
   auto opA = [&](int64_t r, int64_t c) { return t == 'n' ? a + r + c * lda : a + c + r * lda; };
-  const bool forward = (s == 'l') == lower_eff;  // left+lower or right+upper walk from block 0
-  // Block q of the walk (0 first): rows (or columns) [start, start + size).
-  auto blk_start = [&](int64_t q) { return (forward ? q : nblk - 1 - q) * kTrsmNb; };
-  auto blk_size = [&](int64_t q) { return std::min<int64_t>(kTrsmNb, k - blk_start(q)); };
   // X_i <- op(inv_ii) (al W_i)  (left)  or  (al W_i) op(inv_ii)  (right).
   auto solve = [&](int64_t i, int64_t bs, T al) -> pb_status_t {
     const T* invb = inv + (i / kTrsmNb) * kTrsmNb * kTrsmNb;
@@ -496,26 +532,7 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
     return gemm('n', t, M, r1 - r0, c1 - c0, T(-1), b + c0 * ldb, ldb, opA(c0, r0), lda, be, W + r0 * ldw,
                 ldw);
   };
-  // Inside a panel each solved block updates only the panel's later blocks;
-  // the rest of W gets one update per panel.  W is scaled by alpha where it
-  // is first touched.
-  const int64_t panel = k >= kTrsmPanelMinK ? kTrsmPanel : 1;
-  for (int64_t q0 = 0; q0 < nblk; q0 += panel) {
-    const int64_t q1 = std::min<int64_t>(q0 + panel, nblk);
-    const int64_t plo = forward ? blk_start(q0) : blk_start(q1 - 1);
-    const int64_t phi = forward ? blk_start(q1 - 1) + blk_size(q1 - 1) : blk_start(q0) + blk_size(q0);
-    for (int64_t q = q0; q < q1; ++q) {
-      const int64_t i = blk_start(q), bs = blk_size(q);
-      const T al = q == 0 ? alpha : T(1);
-      PB_TRY(solve(i, bs, al));
-      if (forward) PB_TRY(update(i + bs, phi, i, i + bs, al));
-      else PB_TRY(update(plo, i, i, i + bs, al));
-    }
-    const T be = q0 == 0 ? alpha : T(1);
-    if (forward) PB_TRY(update(phi, k, plo, phi, be));
-    else PB_TRY(update(0, plo, plo, phi, be));
-  }
-  return PB_STATUS_SUCCESS;
+  return walk(solve, update);
 }
 
 // ---------------------------------------------------------------------------

@@ -110,6 +110,18 @@ def test_gpu_trsm_panels(handle, torch_cuda, side, uplo, trans, diag):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("side,uplo,trans,diag,M,N,dtype", [
+    ("l", "l", "n", "n", 2048, 512, np.float32), ("l", "u", "n", "n", 2048, 256, np.float32),
+    ("r", "u", "n", "u", 512, 2048, np.float32), ("r", "l", "t", "n", 256, 2048, np.float32),
+    ("l", "l", "t", "n", 2048, 512, np.float64), ("r", "l", "n", "n", 256, 2048, np.float64)])
+def test_gpu_trsm_split_trailing_updates(handle, torch_cuda, side, uplo, trans, diag, M, N, dtype):
+    """Shapes whose narrower trailing updates split K while the full-width
+    update does not (n or m = 512: 256 tiles of 64 x 64 at full width, 192 at
+    1536 rows) or splits less (256: 2-way at full width, 3-way at 1536 rows)."""
+    _check(handle, torch_cuda, dtype, side, uplo, trans, diag, M, N, np.nan, seed=M + 3 * N)
+
+
+@pytest.mark.gpu
 def test_gpu_trsm_argument_errors(handle, torch_cuda):
     import portblas_amd as pb
 
