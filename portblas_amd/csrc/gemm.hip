@@ -369,16 +369,19 @@ constexpr int64_t kTrsmPanelMinK = 2048;
 
 // Inverts the triangular diagonal block `blk` of A (order min(64, k - 64 blk),
 // padded to 64 with the identity) into inv + blk * 64 * 64 (ld 64, same
-// triangle).  Thread c computes column c by substitution; only the stored
-// triangle (and the diagonal unless unit) of A is read.
+// triangle).  Thread c computes column c of the inverse by substitution,
+// right-looking: its 64 running sums stay in registers, and each solved
+// entry x_i updates the sums of the rows still to come with independent
+// FMAs (a left-looking dot product per entry would chain one LDS read per
+// term).  Only the stored triangle (and the diagonal unless unit) of A is read.
 template <typename T>
 __global__ __launch_bounds__(64) void trsm_invert_kernel(int64_t k, const T* __restrict__ a, int64_t lda,
                                                          bool upper, bool unit, T* __restrict__ inv) {
   constexpr int NB = kTrsmNb;
-  // 64 KB for FP64.  Thread c reads L[i][m] (broadcast) and its own column
-  // X[m][c] (consecutive threads, consecutive banks).
-  __shared__ T L[NB][NB];  // L[i][j] = A(i0 + i, i0 + j) in the stored triangle, 0 elsewhere
-  __shared__ T X[NB][NB];  // X[i][c]: the inverse being built
+  // LT[j][i] = A(i0 + i, i0 + j) in the stored triangle, 0 elsewhere (column j
+  // of the block contiguous, read as a broadcast); X[i][c]: the inverse.
+  __shared__ T LT[NB][NB];
+  __shared__ T X[NB][NB];
   const int64_t i0 = int64_t(blockIdx.x) * NB;
   const int bs = int(k - i0 < NB ? k - i0 : NB);
   const int c = threadIdx.x;
@@ -387,20 +390,31 @@ __global__ __launch_bounds__(64) void trsm_invert_kernel(int64_t k, const T* __r
     T v = T(0);
     if (c < bs && j < bs && in_tri && !(unit && c == j)) v = a[(i0 + c) + (i0 + j) * lda];
     if (c == j && (unit || c >= bs)) v = T(1);
-    L[c][j] = v;
+    LT[j][c] = v;
   }
   __syncthreads();
+  T acc[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) acc[r] = (r == c) ? T(1) : T(0);
   if (!upper) {
+#pragma unroll
     for (int i = 0; i < NB; ++i) {
-      T acc = (i == c) ? T(1) : T(0);
-      for (int m = c; m < i; ++m) acc -= L[i][m] * X[m][c];
-      X[i][c] = (i < c) ? T(0) : acc / L[i][i];
+      const T xi = (i < c) ? T(0) : acc[i] / LT[i][i];
+      X[i][c] = xi;
+      if (i >= c) {
+#pragma unroll
+        for (int r = i + 1; r < NB; ++r) acc[r] -= LT[i][r] * xi;
+      }
     }
   } else {
+#pragma unroll
     for (int i = NB - 1; i >= 0; --i) {
-      T acc = (i == c) ? T(1) : T(0);
-      for (int m = i + 1; m <= c; ++m) acc -= L[i][m] * X[m][c];
-      X[i][c] = (i > c) ? T(0) : acc / L[i][i];
+      const T xi = (i > c) ? T(0) : acc[i] / LT[i][i];
+      X[i][c] = xi;
+      if (i <= c) {
+#pragma unroll
+        for (int r = 0; r < i; ++r) acc[r] -= LT[i][r] * xi;
+      }
     }
   }
   __syncthreads();
