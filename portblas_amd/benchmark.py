@@ -6,18 +6,19 @@
 The reference's benchmark suite (benchmark/README.md:111-166) reads one
 parameter set per CSV line, runs each operator many times and reports Google
 Benchmark counters.  This module keeps that interface for the operators of
-the hot path, so the reference's parameter files (benchmark/config_csv/...)
+the hot path and for TRSM, so the reference's parameter files (benchmark/config_csv/...)
 run unchanged and the results compare counter for counter:
 
 * CSV formats and default ranges: common/include/common/common_utils.hpp
   (get_blas1_params :286-306, get_blas2_params :316-, get_blas3_params
   :469-517, get_gemm_batched_params :725-, get_gemm_batched_strided_params
-  :787-, get_reduction_params :900-921; empty lines are skipped, complex
-  alpha/beta pairs keep their real parts);
+  :787-, get_reduction_params :900-921, get_trsm_params :1015-1049; empty
+  lines are skipped, complex alpha/beta pairs keep their real parts);
 * benchmark names: common/include/common/benchmark_names.hpp
   ("BM_<Op><type>/<params>/<mem_type>");
 * counters: common/include/common/blas1_state_counters.hpp,
   blas2_state_counters.hpp:73-100, blas3_state_counters.hpp:33-74 and
+  :226-253 (TRSM, benchmark/portblas/blas3/trsm.cpp), and
   benchmark/portblas/extension/reduction.cpp:41-49 (n_fl_ops,
   bytes_processed, the sizes), common_utils.hpp:1893-1919 (best / total /
   average event and overall times, in ns);
@@ -43,10 +44,10 @@ from typing import Callable
 
 MEM_TYPE = "usm"
 OPS = ("gemm", "gemm_batched", "gemm_batched_strided", "gemv", "dot", "asum", "nrm2", "iamax", "iamin",
-       "reduction")
+       "reduction", "trsm")
 OP_NAMES = {"gemm": "Gemm", "gemm_batched": "Gemm_batched", "gemm_batched_strided": "Gemm_batched_strided",
             "gemv": "Gemv", "dot": "Dot", "asum": "Asum", "nrm2": "Nrm2", "iamax": "Iamax", "iamin": "Iamin",
-            "reduction": "Reduction"}
+            "reduction": "Reduction", "trsm": "Trsm"}
 ELEM_SIZE = {"float": 4, "double": 8}
 
 
@@ -157,9 +158,20 @@ def _p_reduction(v):
     return (_int(v[0]), _int(v[1]))
 
 
+def _p_trsm(v):
+    """side, uplo, trans, diag (first character of each), m, n, alpha
+    (common_utils.hpp:1033-1047)."""
+    if len(v) != 7:
+        raise ValueError("invalid number of parameters (7 expected)")
+    flags = [_cell(x)[:1] for x in v[:4]]
+    if not all(flags):
+        raise ValueError("invalid parameter")
+    return (*flags, _int(v[4]), _int(v[5]), _scalar(v[6]))
+
+
 _PARSERS = {"gemm": _p_gemm, "gemm_batched": _p_gemm_batched, "gemm_batched_strided": _p_gemm_batched_strided,
             "gemv": _p_gemv, "dot": _p_blas1, "asum": _p_blas1, "nrm2": _p_blas1, "iamax": _p_blas1,
-            "iamin": _p_blas1, "reduction": _p_reduction}
+            "iamin": _p_blas1, "reduction": _p_reduction, "trsm": _p_trsm}
 
 
 def default_params(op: str) -> list:
@@ -180,6 +192,9 @@ def default_params(op: str) -> list:
                 for m in (128, 1024, 8192)]
     if op == "reduction":
         return [(r, r) for r in (256, 512, 1024, 2048, 4096, 8192)]
+    if op == "trsm":  # common_utils.hpp:1016-1032: m = n from 512 to 8192 by 4x
+        return [(s, u, t, d, m, m, 1.0) for s in ("l", "r") for u in ("u", "l") for t in ("n", "t")
+                for d in ("u", "n") for m in (512, 2048, 8192)]
     raise ValueError(f"unknown operator '{op}'")
 
 
@@ -206,6 +221,8 @@ def benchmark_names(op: str, dtype: str, p: tuple) -> list:
         args = list(p[:5]) + [p[7], _batch_type_str(p[8])]
     elif op == "gemm_batched_strided":
         args = list(p[:5]) + list(p[7:11])
+    elif op == "trsm":
+        args = list(p[:6])
     else:  # reduction
         return [f"{base}/{p[0]}/{p[1]}/{d}/{MEM_TYPE}" for d in ("inner", "outer")]
     return [f"{base}/" + "/".join(_fmt(a) for a in args) + f"/{MEM_TYPE}"]
@@ -238,6 +255,15 @@ def counters(op: str, dtype: str, p: tuple) -> dict:
         nbytes = (m * k + k * n + m * n + (m * n if beta != 0 else 0)) * batch * es
         c.update(n_fl_ops=flops, bytes_processed=nbytes)
         return c
+    if op == "trsm":
+        # blas3_state_counters.hpp:226-253, kept as written there (for side
+        # 'r' the A term counts m n elements and the B term n (n + 1) / 2).
+        side, m, n = p[0], float(p[4]), float(p[5])
+        tri = (m if side == "l" else n)
+        flops = 2 * tri * (tri + 1) / 2 * (n if side == "l" else m) + m * n
+        read_a = m * (m + 1) / 2 if side == "l" else m * n
+        read_b = n * (n + 1) / 2 if side == "r" else m * n
+        return {"beta": 0.0, "m": m, "n": n, "n_fl_ops": flops, "bytes_processed": (read_a + read_b + m * n) * es}
     rows, cols = float(p[0]), float(p[1])
     return {"rows": rows, "cols": cols, "n_fl_ops": rows * cols, "bytes_processed": rows * cols * es}
 
@@ -295,6 +321,20 @@ def _operator(op: str, dtype: str, p: tuple, h, torch, variant: str | None = Non
             A, B, C = rnd(sa * ma * batch), rnd(sb * mb * batch), rnd(sc * mc * batch)
             call = lambda: pb._gemm_strided_batched(h, ta, tb, m, n, k, alpha, A, lda, sa * ma, B, ldb,  # noqa: E731
                                                     sb * mb, beta, C, m, sc * mc, batch)
+    elif op == "trsm":
+        side, uplo, trans, diag, m, n, alpha = p
+        k = m if side in "lL" else n
+        # Triangle uniform in +-d / k and diagonal d (d in [1, 10] unless unit):
+        # diagonally dominant, as the reference's fill_trsm_matrix
+        # (common_utils.hpp:1758-1781) makes it; the time does not depend on
+        # the values.
+        d = 1.0 if diag in "uU" else 1.0 + 9.0 * float(torch.rand(1, generator=torch.Generator().manual_seed(0)))
+        R = rnd(k * k).view(k, k) * (d / k)  # R[j][i] holds A(i, j): the column-major matrix
+        T = torch.triu(R) if uplo in "lL" else torch.tril(R)
+        T.diagonal().fill_(d)
+        A = T.reshape(-1)
+        B = rnd(m * n)
+        call = lambda: pb._trsm(h, side, uplo, trans, diag, m, n, alpha, A, k, B, m)  # noqa: E731
     else:
         rows, cols = p
         dim = pb.reduction_dim_t[variant]
