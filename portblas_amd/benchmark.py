@@ -6,19 +6,21 @@
 The reference's benchmark suite (benchmark/README.md:111-166) reads one
 parameter set per CSV line, runs each operator many times and reports Google
 Benchmark counters.  This module keeps that interface for the operators of
-the hot path and for TRSM, so the reference's parameter files (benchmark/config_csv/...)
+the hot path and for SYMM and TRSM, so the reference's parameter files (benchmark/config_csv/...)
 run unchanged and the results compare counter for counter:
 
 * CSV formats and default ranges: common/include/common/common_utils.hpp
   (get_blas1_params :286-306, get_blas2_params :316-, get_blas3_params
   :469-517, get_gemm_batched_params :725-, get_gemm_batched_strided_params
-  :787-, get_reduction_params :900-921, get_trsm_params :1015-1049; empty
+  :787-, get_reduction_params :900-921, get_symm_params :933-966,
+  get_trsm_params :1015-1049; empty
   lines are skipped, complex alpha/beta pairs keep their real parts);
 * benchmark names: common/include/common/benchmark_names.hpp
   ("BM_<Op><type>/<params>/<mem_type>");
 * counters: common/include/common/blas1_state_counters.hpp,
   blas2_state_counters.hpp:73-100, blas3_state_counters.hpp:33-74 and
-  :226-253 (TRSM, benchmark/portblas/blas3/trsm.cpp), and
+  :139-167 (SYMM, benchmark/portblas/blas3/symm.cpp) and :226-253 (TRSM,
+  benchmark/portblas/blas3/trsm.cpp), and
   benchmark/portblas/extension/reduction.cpp:41-49 (n_fl_ops,
   bytes_processed, the sizes), common_utils.hpp:1893-1919 (best / total /
   average event and overall times, in ns);
@@ -44,10 +46,10 @@ from typing import Callable
 
 MEM_TYPE = "usm"
 OPS = ("gemm", "gemm_batched", "gemm_batched_strided", "gemv", "dot", "asum", "nrm2", "iamax", "iamin",
-       "reduction", "trsm")
+       "reduction", "symm", "trsm")
 OP_NAMES = {"gemm": "Gemm", "gemm_batched": "Gemm_batched", "gemm_batched_strided": "Gemm_batched_strided",
             "gemv": "Gemv", "dot": "Dot", "asum": "Asum", "nrm2": "Nrm2", "iamax": "Iamax", "iamin": "Iamin",
-            "reduction": "Reduction", "trsm": "Trsm"}
+            "reduction": "Reduction", "symm": "Symm", "trsm": "Trsm"}
 ELEM_SIZE = {"float": 4, "double": 8}
 
 
@@ -158,6 +160,17 @@ def _p_reduction(v):
     return (_int(v[0]), _int(v[1]))
 
 
+def _p_symm(v):
+    """side, uplo (first character of each), m, n, alpha, beta
+    (common_utils.hpp:949-964)."""
+    if len(v) != 6:
+        raise ValueError("invalid number of parameters (6 expected)")
+    flags = [_cell(x)[:1] for x in v[:2]]
+    if not all(flags):
+        raise ValueError("invalid parameter")
+    return (*flags, _int(v[2]), _int(v[3]), _scalar(v[4]), _scalar(v[5]))
+
+
 def _p_trsm(v):
     """side, uplo, trans, diag (first character of each), m, n, alpha
     (common_utils.hpp:1033-1047)."""
@@ -171,7 +184,8 @@ def _p_trsm(v):
 
 _PARSERS = {"gemm": _p_gemm, "gemm_batched": _p_gemm_batched, "gemm_batched_strided": _p_gemm_batched_strided,
             "gemv": _p_gemv, "dot": _p_blas1, "asum": _p_blas1, "nrm2": _p_blas1, "iamax": _p_blas1,
-            "iamin": _p_blas1, "reduction": _p_reduction, "trsm": _p_trsm}
+            "iamin": _p_blas1, "reduction": _p_reduction, "symm": _p_symm,
+            "trsm": _p_trsm}
 
 
 def default_params(op: str) -> list:
@@ -192,6 +206,8 @@ def default_params(op: str) -> list:
                 for m in (128, 1024, 8192)]
     if op == "reduction":
         return [(r, r) for r in (256, 512, 1024, 2048, 4096, 8192)]
+    if op == "symm":  # common_utils.hpp:934-948: m = n from 32 to 8192 by 4x, alpha = beta = 1
+        return [(s, u, m, m, 1.0, 1.0) for s in ("l", "r") for u in ("u", "l") for m in (32, 128, 512, 2048, 8192)]
     if op == "trsm":  # common_utils.hpp:1016-1032: m = n from 512 to 8192 by 4x
         return [(s, u, t, d, m, m, 1.0) for s in ("l", "r") for u in ("u", "l") for t in ("n", "t")
                 for d in ("u", "n") for m in (512, 2048, 8192)]
@@ -221,7 +237,7 @@ def benchmark_names(op: str, dtype: str, p: tuple) -> list:
         args = list(p[:5]) + [p[7], _batch_type_str(p[8])]
     elif op == "gemm_batched_strided":
         args = list(p[:5]) + list(p[7:11])
-    elif op == "trsm":
+    elif op in ("symm", "trsm"):
         args = list(p[:6])
     else:  # reduction
         return [f"{base}/{p[0]}/{p[1]}/{d}/{MEM_TYPE}" for d in ("inner", "outer")]
@@ -255,6 +271,14 @@ def counters(op: str, dtype: str, p: tuple) -> dict:
         nbytes = (m * k + k * n + m * n + (m * n if beta != 0 else 0)) * batch * es
         c.update(n_fl_ops=flops, bytes_processed=nbytes)
         return c
+    if op == "symm":
+        # blas3_state_counters.hpp:139-167, kept as written there (for side
+        # 'r' the product counts 2 n^3 flops).
+        side, m, n, beta = p[0], float(p[2]), float(p[3]), p[5]
+        flops = (2 * m * m * n if side == "l" else 2 * n * n * n) + (2 * m * n if beta != 0 else 0)
+        read_a = m * (m + 1) / 2 if side == "l" else n * (n + 1) / 2
+        nbytes = ((2 if beta != 0 else 1) * m * n + m * n + read_a) * es
+        return {"beta": float(beta), "m": m, "n": n, "n_fl_ops": flops, "bytes_processed": nbytes}
     if op == "trsm":
         # blas3_state_counters.hpp:226-253, kept as written there (for side
         # 'r' the A term counts m n elements and the B term n (n + 1) / 2).
@@ -321,6 +345,11 @@ def _operator(op: str, dtype: str, p: tuple, h, torch, variant: str | None = Non
             A, B, C = rnd(sa * ma * batch), rnd(sb * mb * batch), rnd(sc * mc * batch)
             call = lambda: pb._gemm_strided_batched(h, ta, tb, m, n, k, alpha, A, lda, sa * ma, B, ldb,  # noqa: E731
                                                     sb * mb, beta, C, m, sc * mc, batch)
+    elif op == "symm":
+        side, uplo, m, n, alpha, beta = p
+        k = m if side in "lL" else n
+        A, B, C = rnd(k * k), rnd(m * n), rnd(m * n)
+        call = lambda: pb._symm(h, side, uplo, m, n, alpha, A, k, B, m, beta, C, m)  # noqa: E731
     elif op == "trsm":
         side, uplo, trans, diag, m, n, alpha = p
         k = m if side in "lL" else n

@@ -1,7 +1,8 @@
 """The portBLAS benchmark front end (portblas_amd.benchmark): CSV parameter
 files in the reference's formats (benchmark/README.md:111-166), its default
 ranges, benchmark names and counters — on the CPU — and one run of every
-operator on the GPU (TRSM included: benchmark/portblas/blas3/trsm.cpp)."""
+operator on the GPU (SYMM and TRSM included: benchmark/portblas/blas3/symm.cpp,
+trsm.cpp)."""
 from __future__ import annotations
 
 import json
@@ -35,11 +36,13 @@ def test_other_formats(tmp_path):
     assert bm.parse_csv(_csv(tmp_path, "513,8195\n"), "reduction") == [(513, 8195)]
     assert bm.parse_csv(_csv(tmp_path, "l,u,n,n,512,256,1.5\nr,l,t,u,64,32,1\n"), "trsm") == [
         ("l", "u", "n", "n", 512, 256, 1.5), ("r", "l", "t", "u", 64, 32, 1.0)]
+    assert bm.parse_csv(_csv(tmp_path, "r,l,64,32,1.5,0\n"), "symm") == [("r", "l", 64, 32, 1.5, 0.0)]
 
 
 @pytest.mark.parametrize("op,text", [("gemm", "n,n,42,42\n"), ("gemm", "x,n,4,4,4,1,0\n"), ("dot", "12,3\n"),
                                      ("gemm_batched", "n,n,4,4,4,1,0,2,diagonal\n"), ("reduction", ""),
-                                     ("trsm", "l,u,n,n,512,512\n"), ("trsm", "l,,n,n,4,4,1\n")])
+                                     ("trsm", "l,u,n,n,512,512\n"), ("trsm", "l,,n,n,4,4,1\n"),
+                                     ("symm", "l,u,4,4,1\n")])
 def test_bad_lines_are_reported(tmp_path, op, text):
     with pytest.raises(bm.CsvError):
         bm.parse_csv(_csv(tmp_path, text), op)
@@ -56,6 +59,8 @@ def test_default_ranges_match_the_reference():
     t = bm.default_params("trsm")
     assert len(t) == 16 * 3 and {p[4] for p in t} == {512, 2048, 8192} and all(p[4] == p[5] for p in t)
     assert t[0] == ("l", "u", "n", "u", 512, 512, 1.0)
+    y = bm.default_params("symm")
+    assert len(y) == 4 * 5 and [p[2] for p in y[:5]] == [32, 128, 512, 2048, 8192] and y[0][4:] == (1.0, 1.0)
 
 
 def test_names():
@@ -71,6 +76,8 @@ def test_names():
         "BM_Reduction<float>/64/32/inner/usm", "BM_Reduction<float>/64/32/outer/usm"]
     assert bm.benchmark_names("trsm", "float", ("l", "u", "n", "u", 512, 256, 1.0)) == [
         "BM_Trsm<float>/l/u/n/u/512/256/usm"]
+    assert bm.benchmark_names("symm", "double", ("r", "l", 32, 64, 1.0, 0.5)) == [
+        "BM_Symm<double>/r/l/32/64/1/0.5/usm"]
 
 
 def test_counters_worked_examples():
@@ -94,6 +101,12 @@ def test_counters_worked_examples():
     assert c["n_fl_ops"] == 72 and c["bytes_processed"] == (10 + 12 + 12) * 4
     c = bm.counters("trsm", "double", ("r", "u", "n", "n", 4, 3, 1.0))
     assert c["n_fl_ops"] == 60 and c["bytes_processed"] == (12 + 6 + 12) * 8
+    # symm m = 4, n = 3: side 'l', beta 1: 2 * 16 * 3 + 2 * 12 flops, B and C read, C written, 10 of A;
+    # side 'r', beta 0: 2 * 3^3 flops (as the reference counts), 6 of A
+    c = bm.counters("symm", "float", ("l", "u", 4, 3, 1.0, 1.0))
+    assert c["n_fl_ops"] == 120 and c["bytes_processed"] == (24 + 12 + 10) * 4
+    c = bm.counters("symm", "float", ("r", "u", 4, 3, 1.0, 0.0))
+    assert c["n_fl_ops"] == 54 and c["bytes_processed"] == (12 + 12 + 6) * 4
 
 
 @pytest.mark.gpu
@@ -101,7 +114,8 @@ def test_gpu_every_operator_runs(tmp_path, torch_cuda):
     files = {"gemm": "n,n,64,32,48,1,0\nt,t,33,17,65,1.5,0.5\n", "gemm_batched": "n,t,16,16,16,1,0,4,strided\n",
              "gemm_batched_strided": "n,n,64,64,64,1,0,32,1,1,1\n", "gemv": "n,100,200,1,0\nt,100,200,2,1\n",
              "dot": "4096\n", "asum": "4096\n", "nrm2": "4096\n", "iamax": "4096\n", "iamin": "4096\n",
-             "reduction": "64,32\n", "trsm": "l,l,n,n,100,70,1\nr,u,t,u,50,130,2\n"}
+             "reduction": "64,32\n", "trsm": "l,l,n,n,100,70,1\nr,u,t,u,50,130,2\n",
+             "symm": "l,u,100,70,1,0.5\nr,l,50,130,2,0\n"}
     for op, text in files.items():
         out = tmp_path / f"{op}.json"
         assert bm.main([op, f"--csv-param={_csv(tmp_path, text, op + '.csv')}", f"--benchmark_out={out}",
