@@ -35,6 +35,7 @@ pb_status_t enter(pb_handle_t h) {
 
 pb_status_t workspace(pb_handle_t h, size_t bytes, void** ptr) {
   bytes = align_up(bytes < 256 ? 256 : bytes, 256);
+  std::lock_guard<std::mutex> lock(h->mu);
   if (bytes <= h->workspace_bytes) {
     *ptr = h->workspace;
     return PB_STATUS_SUCCESS;
@@ -51,9 +52,12 @@ pb_status_t workspace(pb_handle_t h, size_t bytes, void** ptr) {
   size_t want = bytes;
   if (want < h->workspace_bytes + h->workspace_bytes / 2)
     want = align_up(h->workspace_bytes + h->workspace_bytes / 2, 256);
-  if (h->workspace) {
+  if (h->ws) {
+    // Work already queued on the stream may still use the old buffer.  Drop
+    // the handle's reference only after it has finished; graphs captured
+    // with the old buffer keep it alive until they are destroyed.
     PB_CHECK_HIP(hipStreamSynchronize(h->stream));
-    PB_CHECK_HIP(hipFree(h->workspace));
+    h->ws.reset();
     h->workspace = nullptr;
     h->workspace_bytes = 0;
   }
@@ -64,6 +68,7 @@ pb_status_t workspace(pb_handle_t h, size_t bytes, void** ptr) {
     return set_error(PB_STATUS_ALLOC_FAILED, "hipMalloc(%zu) failed: %s", want,
                      hipGetErrorString(e));
   }
+  h->ws = std::make_shared<pb_device_buffer>(p, want);
   h->workspace = p;
   h->workspace_bytes = want;
   *ptr = p;
@@ -76,6 +81,12 @@ unsigned int* counters(pb_handle_t h, int n) {
 }
 
 }  // namespace pb
+
+// hipFree waits for the device's outstanding work, so a graph destroyed right
+// after its last launch does not free memory that launch still uses.
+pb_device_buffer::~pb_device_buffer() {
+  if (ptr) (void)hipFree(ptr);
+}
 
 using namespace pb;
 
@@ -119,10 +130,12 @@ pb_status_t pb_handle_create(void* stream, pb_handle_t* out) {
   if (e == hipSuccess)
     e = hipMemset(h->counters, 0, kCounterSlots * sizeof(unsigned int));
   if (e == hipSuccess) e = hipMalloc(&h->scalar_slot, 256);
+  if (e == hipSuccess) e = hipEventCreateWithFlags(&h->switch_event, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     if (h->counters) (void)hipFree(h->counters);
     if (h->scalar_slot) (void)hipFree(h->scalar_slot);
+    if (h->switch_event) (void)hipEventDestroy(h->switch_event);
     delete h;
     return set_error(PB_STATUS_DEVICE_ERROR, "counter arena setup failed: %s",
                      hipGetErrorString(e));
@@ -136,9 +149,10 @@ pb_status_t pb_handle_destroy(pb_handle_t h) {
   if (!h) return PB_STATUS_SUCCESS;
   PB_TRY(enter(h));
   hipError_t e = hipStreamSynchronize(h->stream);
-  if (h->workspace) (void)hipFree(h->workspace);
+  h->ws.reset();  // freed now unless a live graph still holds it
   if (h->counters) (void)hipFree(h->counters);
   if (h->scalar_slot) (void)hipFree(h->scalar_slot);
+  if (h->switch_event) (void)hipEventDestroy(h->switch_event);
   delete h;
   if (e != hipSuccess)
     return set_error(PB_STATUS_DEVICE_ERROR, "stream sync at destroy: %s",
@@ -149,7 +163,15 @@ pb_status_t pb_handle_destroy(pb_handle_t h) {
 pb_status_t pb_handle_set_stream(pb_handle_t h, void* stream) {
   clear_error();
   PB_TRY(enter(h));
-  h->stream = static_cast<hipStream_t>(stream);
+  hipStream_t next = static_cast<hipStream_t>(stream);
+  if (next != h->stream) {
+    // The workspace and the counter arena are shared by every call: work
+    // still queued on the old stream must finish before the new stream's
+    // first kernel may touch them.
+    PB_CHECK_HIP(hipEventRecord(h->switch_event, h->stream));
+    PB_CHECK_HIP(hipStreamWaitEvent(next, h->switch_event, 0));
+  }
+  h->stream = next;
   return PB_STATUS_SUCCESS;
 }
 
@@ -195,6 +217,9 @@ pb_status_t pb_handle_record_event(pb_handle_t h, void* ev) {
 struct pb_graph_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  // The workspace the captured kernels address; kept alive while the graph
+  // lives, whatever the handle's workspace does in the meantime.
+  std::shared_ptr<pb_device_buffer> ws;
 };
 
 pb_status_t pb_graph_capture_begin(pb_handle_t h) {
@@ -221,6 +246,10 @@ pb_status_t pb_graph_capture_end(pb_handle_t h, pb_graph_t* out) {
     return set_error(PB_STATUS_DEVICE_ERROR, "graph capture failed: %s",
                      hipGetErrorString(e));
   }
+  {
+    std::lock_guard<std::mutex> lock(h->mu);
+    g->ws = h->ws;
+  }
   *out = g;
   return PB_STATUS_SUCCESS;
 }
@@ -238,7 +267,7 @@ pb_status_t pb_graph_destroy(pb_graph_t g) {
   if (!g) return PB_STATUS_SUCCESS;
   if (g->exec) (void)hipGraphExecDestroy(g->exec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
-  delete g;
+  delete g;  // drops the graph's hold on its workspace
   return PB_STATUS_SUCCESS;
 }
 

@@ -7,6 +7,7 @@
 #include <cstdarg>
 #include <cstdint>
 #include <cstdio>
+#include <memory>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -66,17 +67,36 @@ struct pb_selector_state {
   std::vector<float> mean, stdv;
 };
 
+// One workspace allocation.  The handle holds the current one; every graph
+// captured on the handle holds the one its kernels were recorded with, so a
+// later growth of the handle's workspace never frees memory that a live graph
+// still reads and writes (the buffer is freed when its last holder lets go).
+struct pb_device_buffer {
+  void* ptr = nullptr;
+  size_t bytes = 0;
+  pb_device_buffer(void* p, size_t n) : ptr(p), bytes(n) {}
+  ~pb_device_buffer();
+  pb_device_buffer(const pb_device_buffer&) = delete;
+  pb_device_buffer& operator=(const pb_device_buffer&) = delete;
+};
+
 struct pb_handle_s {
   hipStream_t stream = nullptr;
   int device = 0;
   int num_cus = 0;
   int max_threads = 1024;
-  void* workspace = nullptr;
+  void* workspace = nullptr;         // == ws->ptr (kept for the kernels' plain access)
   size_t workspace_bytes = 0;
+  std::shared_ptr<pb_device_buffer> ws;
   unsigned int* counters = nullptr;  // kCounterSlots zero-initialised slots
   void* scalar_slot = nullptr;       // 256 bytes for the _sync BLAS-1 results
+  hipEvent_t switch_event = nullptr; // orders a stream switch after the old stream's work
   size_t counter_cursor = 0;
   bool capturing = false;
+  // Guards the workspace pool (growth and hand-out), as the reference's
+  // Temp_Mem_Pool guards its pool (include/sb_handle/temp_memory_pool.h:98-108).
+  // Calls that overlap on one handle are still unsupported: they share the
+  // stream, the workspace contents and the counter arena.
   std::mutex mu;
   pb_selector_state selector;
 };
