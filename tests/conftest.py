@@ -63,3 +63,31 @@ def to_dev(torch, a):
 
 U32 = 2.0 ** -24  # FP32 unit roundoff
 U64 = 2.0 ** -53  # FP64 unit roundoff
+
+
+def cfg1_inputs(golden):
+    import oracle
+
+    """The 1024^3 CPU-runnable configuration: A, B, C consecutive on stream 0."""
+    ta, tb, m, n, k, lda, ldb, ldc = (int(v) for v in golden["cfg1_args"])
+    alpha, beta = (float(v) for v in golden["cfg1_scal"])
+    u = oracle.uniform(int(golden["cfg1_seed"]), m * k + k * n + m * n)
+    return chr(ta), chr(tb), m, n, k, lda, ldb, ldc, alpha, beta, u[: m * k], u[m * k: m * k + k * n], \
+        u[m * k + k * n:]
+
+
+def check_cfg1(golden, got, slack):
+    """Entries and checksums of a 1024^3 result against the CBLAS fixture:
+    both carry at most one FP32 forward error, so each compare allows
+    `slack` x the bound."""
+    import oracle
+
+    ta, tb, m, n, k, lda, ldb, ldc, alpha, beta, A, B, C = cfg1_inputs(golden)
+    G = colmajor(got, m, n, ldc).astype(np.float64)
+    absprod = colmajor(oracle.gemm_absprod_f64(ta, tb, m, n, k, A, lda, B, ldb), m, n, m)
+    tol = slack * (1.01 * (k + 2) * U32 * abs(alpha) * absprod + 2 * U32 * abs(beta) * np.abs(colmajor(C, m, n, ldc)) + 1e-30)
+    rows, cols = golden["cfg1_rows"], golden["cfg1_cols"]
+    e = np.abs(G[np.ix_(rows, cols)] - golden["cfg1_sampled"].astype(np.float64))
+    assert np.all(e <= tol[np.ix_(rows, cols)]), f"sampled: max err/tol {np.max(e / tol[np.ix_(rows, cols)]):.3g}"
+    assert np.all(np.abs(G.sum(axis=0) - golden["cfg1_colsum"]) <= tol.sum(axis=0))
+    assert np.all(np.abs(G.sum(axis=1) - golden["cfg1_rowsum"]) <= tol.sum(axis=1))

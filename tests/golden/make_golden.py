@@ -62,6 +62,31 @@ def gemm_cases(d):
         d[p + "out"] = Cres.T.reshape(-1).astype(np.float32)
 
 
+def gemm_cfg1(d):
+    """The reference's CPU-runnable configuration (1024^3, 'n','n', lda = ldb =
+    ldc = 1024, alpha 1.5, beta 0.5) on one splitmix64 stream, seed 0: A is
+    its first 1024^2 values, B the next, C the next.  Stored: 64 x 64 sampled
+    entries of the CBLAS result (rows and columns include the ends and the
+    128-multiples) and its FP64 row and column sums."""
+    m = n = k = 1024
+    u = oracle.uniform(0, m * k + k * n + m * n)
+    A, B, C = u[: m * k], u[m * k: m * k + k * n], u[m * k + k * n:]
+    out = sblas.sgemm(1.5, colmajor(A, m, k, m), colmajor(B, k, n, k), beta=0.5,
+                      c=np.asfortranarray(colmajor(C, m, n, m)))
+    rng = np.random.default_rng(1024)
+    fixed = [0, 1, 127, 128, 511, 512, 895, 896, 1022, 1023]
+    rows = np.array(sorted(set(fixed) | set(rng.choice(m, 64, replace=False).tolist())))[:64]
+    cols = np.array(sorted(set(fixed) | set(rng.choice(n, 64, replace=False).tolist())))[:64]
+    d["cfg1_args"] = np.array([ord("n"), ord("n"), m, n, k, m, k, m], dtype=np.int64)
+    d["cfg1_scal"] = np.array([1.5, 0.5])
+    d["cfg1_seed"] = np.int64(0)
+    d["cfg1_rows"] = rows.astype(np.int64)
+    d["cfg1_cols"] = cols.astype(np.int64)
+    d["cfg1_sampled"] = out[np.ix_(rows, cols)].astype(np.float32)
+    d["cfg1_colsum"] = out.astype(np.float64).sum(axis=0)
+    d["cfg1_rowsum"] = out.astype(np.float64).sum(axis=1)
+
+
 def gemv_cases(d):
     cases = [("n", 11, 14, 1.0, 0.0, 1, 2, 3), ("t", 11, 14, 1.5, 1.5, 2, 2, 3),
              ("n", 1023, 1010, 1.0, 1.5, 1, 1, 1), ("t", 1023, 1010, 1.0, 0.0, 1, 1, 1)]
@@ -155,6 +180,7 @@ def main():
     # other cases are regenerated from their seeds by the tests).
     d["uniform_seed0_head"] = oracle.uniform(0, 16)
     gemm_cases(d)
+    gemm_cfg1(d)
     gemv_cases(d)
     blas1_cases(d)
     iaminmax_patterns(d)

@@ -8,7 +8,7 @@ import numpy as np
 import pytest
 
 import oracle
-from conftest import U32, U64, colmajor, to_dev
+from conftest import U32, U64, cfg1_inputs, check_cfg1, colmajor, to_dev
 
 pytestmark = pytest.mark.gpu
 
@@ -98,10 +98,23 @@ def test_gemm_beta_zero_never_reads_c(handle, torch_cuda):
         run_case(handle, torch_cuda, ta, "n", 256, 256, 64, 1.0, 0.0, poison_c=True, seed=4)
 
 
-def test_gemm_cfg1_1024(handle, torch_cuda):
+def test_gemm_cfg1_1024(handle, torch_cuda, golden):
     """The reference's CPU-runnable configuration: 1024^3, 'n','n', ld 1024,
-    alpha 1.5, beta 0.5, inputs U[-1,1] from seed 0."""
-    run_case(handle, torch_cuda, "n", "n", 1024, 1024, 1024, 1.5, 0.5, seed=0)
+    alpha 1.5, beta 0.5, inputs U[-1,1] from seed 0, checked against the
+    committed CBLAS sgemm entries and checksums and against the FP64 oracle;
+    then the same shape on other seeds through run_case."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    ta, tb, m, n, k, lda, ldb, ldc, alpha, beta, A, B, C = cfg1_inputs(golden)
+    dC = to_dev(torch, C)
+    pb._gemm(handle, ta, tb, m, n, k, alpha, to_dev(torch, A), lda, to_dev(torch, B), ldb, beta, dC, ldc)
+    got = dC.cpu().numpy()
+    check_cfg1(golden, got, 2.0)
+    ref = oracle.gemm_f64(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    absprod = oracle.gemm_absprod_f64(ta, tb, m, n, k, A, lda, B, ldb)
+    assert np.all(np.abs(got - ref) <= bound(k, alpha, absprod, beta, np.abs(C), U32))
+    run_case(handle, torch, "n", "n", 1024, 1024, 1024, 1.5, 0.5, seed=0)
 
 
 @pytest.mark.parametrize("ta", "nt")

@@ -9,7 +9,7 @@ import pytest
 from scipy.linalg import blas as sblas
 
 import oracle
-from conftest import U32, U64, colmajor
+from conftest import U32, U64, cfg1_inputs, check_cfg1, colmajor
 
 RNG = np.random.default_rng(1234)
 
@@ -116,6 +116,21 @@ def test_portblas_cpu_gemm_restatement(golden):
         # The (beta/alpha) C pre-scale adds one rounding of |beta C| per element.
         tol = gemm_bound(k, alpha, absprod, beta, np.abs(colmajor(C, m, n, ldc)), U32) * 2
         assert np.all(np.abs(colmajor(got, m, n, ldc) - colmajor(ref, m, n, ldc)) <= tol)
+
+
+def test_cfg1_oracle_and_cpu_path_vs_cblas(golden):
+    """cfg1 (1024^3, alpha 1.5, beta 0.5, seed 0) checked against CBLAS
+    sgemm: the FP64 oracle and the restated portBLAS CPU schedule both agree
+    with the committed CBLAS entries and checksums; a transposed operand in
+    the oracle call is caught."""
+    ta, tb, m, n, k, lda, ldb, ldc, alpha, beta, A, B, C = cfg1_inputs(golden)
+    ref = oracle.gemm_f64(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    check_cfg1(golden, ref, 1.0)
+    cpu = oracle.pb_sgemm(ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    check_cfg1(golden, cpu, 3.0)
+    wrong = oracle.gemm_f64(ta, "t", m, n, k, alpha, A, lda, B, ldb, beta, C, ldc)
+    with pytest.raises(AssertionError):
+        check_cfg1(golden, wrong, 1.0)
 
 
 def test_gemm_closed_forms():
