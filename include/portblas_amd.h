@@ -117,6 +117,21 @@ pb_status_t pb_dgemm(pb_handle_t h, char transa, char transb, int64_t m, int64_t
                      int64_t k, double alpha, const double* a, int64_t lda,
                      const double* b, int64_t ldb, double beta, double* c, int64_t ldc);
 
+/* Allocation-free pb_sgemm.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
+ * validation, plan, kernels and results as pb_sgemm, but the split-K partial
+ * tiles live in a 16 MiB array the library holds on each GPU from load to
+ * process exit, and the call never grows the handle workspace and never
+ * allocates.  A plan needing more partials than the array holds returns
+ * PB_STATUS_ALLOC_FAILED (a plain FP32 plan needs at most 8 MiB on 256 CUs).
+ * Every call on one GPU shares that GPU's array: calls that overlap on one
+ * GPU (two streams, two threads) race; serialize them. */
+pb_status_t pb_sgemm_noalloc(pb_handle_t h, char transa, char transb, int64_t m,
+                             int64_t n, int64_t k, float alpha, const float* a,
+                             int64_t lda, const float* b, int64_t ldb, float beta,
+                             float* c, int64_t ldc);
+
 /* blas::_gemm_batched, blas3_interface.h:97-109; default strides
  * gemm_interface.hpp:202-226 (strided: A stride = (ta=='n' ? K : M) * lda,
  * B stride = (tb=='n' ? N : K) * ldb, C stride = N * ldc).  Interleaved:

@@ -37,6 +37,7 @@ SIGNATURES: dict[str, list] = {
     "pb_abi_version": [],
     "pb_sgemm": [_p, _ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
     "pb_dgemm": [_p, _ch, _ch, _i64, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64],
+    "pb_sgemm_noalloc": [_p, _ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64],
     "pb_sgemm_batched": [_p, _ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _p, _i64, _f32, _p, _i64, _i64, _i32],
     "pb_dgemm_batched": [_p, _ch, _ch, _i64, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p, _i64, _i64, _i32],
     "pb_sgemm_strided_batched": [_p, _ch, _ch, _i64, _i64, _i64, _f32, _p, _i64, _i64, _p, _i64, _i64,

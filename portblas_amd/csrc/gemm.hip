@@ -674,9 +674,68 @@ pb_status_t cgemm_entry(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, i
 
 }  // namespace pbgemm
 
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free SGEMM (pb_sgemm_noalloc): the split-K partial
+// tiles, in device memory the library owns from load to process exit (one
+// copy per GPU).  A plain FP32 GEMM splits K only while its 64 x 64 tiles are
+// fewer than the compute units, into ceil(CUs / tiles) splits, so it never
+// writes more than 2 * CUs - 1 partial tiles of 16 KiB: 8 MiB on 256 CUs.
+// Every split writes its whole tile before the fold reads it, so nothing here
+// needs clearing between calls.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace pbgemm {
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_addr[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device (which must be `device`),
+// looked up on the first call per device and kept.  The array's name in host
+// code is only a host-side stand-in, never a device address.
+pb_status_t scratch_address(int device, void** out) {
+  PB_REQUIRE(device >= 0 && device < kMaxDevices, "device %d out of range", device);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_addr[device]) {
+    void* p = nullptr;
+    PB_CHECK_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_addr[device] = p;
+  }
+  *out = g_scratch_addr[device];
+  return PB_STATUS_SUCCESS;
+}
+}  // namespace
+}  // namespace pbgemm
+
 using namespace pbgemm;
 
 extern "C" {
+
+pb_status_t pb_sgemm_noalloc(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                             float alpha, const float* a, int64_t lda, const float* b,
+                             int64_t ldb, float beta, float* c, int64_t ldc) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  GemmCall<float> g;
+  bool done = false;
+  PB_TRY(gemm_front<float>(h, ta, tb, m, n, k, alpha, a, lda, 0, b, ldb, 0, beta, c, ldc, 0, 1,
+                           PB_BATCH_STRIDED, g, done));
+  if (done) return PB_STATUS_SUCCESS;
+  if (g.alpha == 0.f)
+    return alpha_zero<float>(h, g.M, g.N, g.beta, g.c, g.ldc, g.sc, g.batch, g.interleaved);
+  // Same plan and kernels as pb_sgemm; only the partials' home differs.
+  const size_t need = plain_partial_bytes<float>(h, g);
+  if (need > kNoAllocScratchBytes)
+    return pb::set_error(PB_STATUS_ALLOC_FAILED,
+                         "pb_sgemm_noalloc: %zu bytes of split-K partials exceed the %zu-byte "
+                         "scratch",
+                         need, kNoAllocScratchBytes);
+  void* part = nullptr;
+  PB_TRY(scratch_address(h->device, &part));
+  return run_plain<float>(h, g, false, static_cast<float*>(part));
+}
 
 pb_status_t pb_cgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, const float* alpha,
                      const float* a, int64_t lda, const float* b, int64_t ldb, const float* beta, float* c,
