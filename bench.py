@@ -51,10 +51,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--quick", action="store_true", help="headline + roofline only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--pmc-probe", nargs="?", const="bf16x3", choices=("bf16x3", "dgemm", "grouped"), default=None,
+    ap.add_argument("--pmc-probe", nargs="?", const="bf16x3", choices=("bf16x3", "dgemm", "grouped", "simt"), default=None,
                     help="for rocprofv3 --pmc: one 256 MiB device copy (the counter scale), then two "
-                         "launches of the BF16x3 kernel on packed 8192^3 operands (or of DGEMM 4096^3); "
-                         "prints no JSON")
+                         "launches of the BF16x3 kernel on packed 8192^3 operands (or of DGEMM 3072^3, "
+                         "the grouped set, or the FP32 SIMT GEMM at 4096^3); prints no JSON")
     return ap.parse_args()
 
 
@@ -102,7 +102,8 @@ def pmc_probe(torch, pb, h, which="bf16x3"):
     scale, then the BF16x3 kernel alone on packed 8192^3 operands (or
     DGEMM 3072^3, small enough that SQ_VALU_MFMA_BUSY_CYCLES, summed over
     the 1024 SIMDs, stays below its 2^32 limit; or the 512-problem grouped
-    set), twice."""
+    set; or the FP32 SIMT GEMM at 4096^3, whose FP32 FMA instruction count
+    tells packed from non-packed FMAs), twice."""
     src = torch.ones(1 << 26, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -116,6 +117,16 @@ def pmc_probe(torch, pb, h, which="bf16x3"):
         Cd = torch.zeros(D * D, dtype=torch.float64, device="cuda")
         for _ in range(2):
             pb._gemm(h, "n", "n", D, D, D, 1.0, Ad, D, Bd, D, 0.0, Cd, D)
+            torch.cuda.synchronize()
+        log("pmc probe done")
+        return
+    if which == "simt":
+        D = 4096
+        As = torch.rand(D * D, device="cuda")
+        Bs = torch.rand(D * D, device="cuda")
+        Cs = torch.zeros(D * D, device="cuda")
+        for _ in range(2):
+            pb._gemm_ex(h, pb.gemm_precision_t.fp32_simt, "n", "n", D, D, D, 1.0, As, D, Bs, D, 0.0, Cs, D)
             torch.cuda.synchronize()
         log("pmc probe done")
         return
