@@ -1,21 +1,26 @@
 """Benchmark of the portBLAS hot path on one MI355X.
 
 Headline (the JSON `value`): FP32-accurate SGEMM at M = N = K = 8192 through
-BF16x3 split precision on MFMA (pb_gemm_ex, PB_GEMM_BF16X3), in FP32-equivalent
-TFLOP/s (2 M N K per step).  One step = one full call: the split/pack pre-pass
-and the 3-term BF16 MFMA GEMM, inputs resident in HBM.  The same JSON line
-carries the rest of the headline metric and its configurations:
+three-term split precision on MFMA, in FP32-equivalent TFLOP/s (2 M N K per
+step), for the split format that holds the max componentwise error vs FP64 at
+or below the plain FP32 GEMM's: FP16x3 (pb_gemm_ex, PB_GEMM_FP16X3: hi/lo
+halves of a power-of-two-scaled operand, 22 significand bits).  BF16x3 (16
+bits) runs at the same speed but misses that condition; its figures stay in
+the line.  One step = one full call: the amax and split pre-passes and the
+3-term MFMA GEMM, inputs resident in HBM.  The same JSON line carries the
+rest of the headline metric and its configurations:
 
-* the plain FP32 MFMA SGEMM on the same inputs and the BF16x3 speedup;
+* the plain FP32 MFMA SGEMM on the same inputs and each split's speedup;
 * the precision study: max / mean componentwise and normwise error of FP32
-  MFMA, FP32 SIMT, BF16x3, BF16 and FP8-e4m3 against the FP64 GPU GEMM
-  (pb_dgemm) on N(0,1) and log-uniform [2^-8, 2^8] inputs, and DGEMM 4096^3;
+  MFMA, FP32 SIMT, FP16x3, BF16x3, BF16x4, BF16 and FP8-e4m3 against the FP64
+  GPU GEMM (pb_dgemm) on N(0,1) and log-uniform [2^-8, 2^8] inputs, and DGEMM
+  4096^3;
 * SGEMV 8192^2 'n'/'t' and sdot/sasum/snrm2/isamax at n = 2^26 in GB/s and as
   a fraction of 8 TB/s (cache flushed before every timed call), and the
   streaming saxpy/scopy/sscal/sswap on the same vectors;
 * the strided batch (4096 x 64^3) and the grouped set (512 uneven BF16
   problems, one persistent launch, direct and graph-replayed);
-* `roofline` for the dominant kernel (the BF16 MFMA GEMM of the BF16x3 step,
+* `roofline` for the dominant kernel (the FP16 MFMA GEMM of the FP16x3 step,
   timed alone with HIP events on pre-split operands) and `cpu_baseline` (the
   oracle's FP32 restatement of portBLAS's CPU GEMM schedule on the host).
 
@@ -35,7 +40,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-PEAK = {"bf16": 2500.0, "fp8": 5000.0, "fp32": 157.3, "fp64": 78.6}  # dense TFLOP/s
+PEAK = {"bf16": 2500.0, "fp16": 2500.0, "fp8": 5000.0, "fp32": 157.3, "fp64": 78.6}  # dense TFLOP/s
 METRIC = "SGEMM TFLOP/s, BF16x3-MFMA vs plain FP32 at equal FP64 error; GEMV/dot % HBM"
 
 
@@ -51,9 +56,10 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--quick", action="store_true", help="headline + roofline only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
-    ap.add_argument("--pmc-probe", nargs="?", const="bf16x3", choices=("bf16x3", "dgemm", "grouped", "simt"), default=None,
+    ap.add_argument("--pmc-probe", nargs="?", const="fp16x3",
+                    choices=("fp16x3", "bf16x3", "dgemm", "grouped", "simt"), default=None,
                     help="for rocprofv3 --pmc: one 256 MiB device copy (the counter scale), then two "
-                         "launches of the BF16x3 kernel on packed 8192^3 operands (or of DGEMM 3072^3, "
+                         "launches of the FP16x3 (or BF16x3) kernel on packed 8192^3 operands (or of DGEMM 3072^3, "
                          "the grouped set, or the FP32 SIMT GEMM at 4096^3); prints no JSON")
     return ap.parse_args()
 
@@ -86,20 +92,20 @@ class Timer:
 
 
 def committed_traffic():
-    """HBM bytes per launch of the BF16x3 kernel at 8192^3, from the
+    """HBM bytes per launch of the FP16x3 kernel at 8192^3, from the
     committed rocprofv3 --pmc summary (FETCH_SIZE + WRITE_SIZE, each scaled
     by the same counter on a copy of known size); None if absent."""
     import re
-    path = Path(__file__).resolve().parent / "profiles" / "mt_kernel_bf16x3_8192_pmc.md"
+    path = Path(__file__).resolve().parent / "profiles" / "mt_kernel_fp16x3_8192_pmc.md"
     if not path.exists():
         return None
     m = re.search(r"traffic_bytes_per_launch\s*=\s*([0-9.eE+]+)", path.read_text())
     return float(m.group(1)) if m else None
 
 
-def pmc_probe(torch, pb, h, which="bf16x3"):
+def pmc_probe(torch, pb, h, which="fp16x3"):
     """A plain 256 MiB copy, whose FETCH_SIZE / WRITE_SIZE give the counters'
-    scale, then the BF16x3 kernel alone on packed 8192^3 operands (or
+    scale, then the FP16x3 or BF16x3 kernel alone on packed 8192^3 operands (or
     DGEMM 3072^3, small enough that SQ_VALU_MFMA_BUSY_CYCLES, summed over
     the 1024 SIMDs, stays below its 2^32 limit; or the 512-problem grouped
     set; or the FP32 SIMT GEMM at 4096^3, whose FP32 FMA instruction count
@@ -146,6 +152,20 @@ def pmc_probe(torch, pb, h, which="bf16x3"):
     C = torch.zeros(N * N, device="cuda")
     ah = torch.empty(N * N, dtype=torch.int16, device="cuda")
     al, bh, bl = torch.empty_like(ah), torch.empty_like(ah), torch.empty_like(ah)
+    if which == "fp16x3":
+        fac = torch.empty(2, device="cuda")
+        pb.check(L.pb_pack_fp16x3(h.handle, 0, b"n", N, N, A.data_ptr(), N, ah.data_ptr(), al.data_ptr(), N,
+                                  fac.data_ptr()))
+        pb.check(L.pb_pack_fp16x3(h.handle, 1, b"n", N, N, B.data_ptr(), N, bh.data_ptr(), bl.data_ptr(), N,
+                                  fac.data_ptr() + 4))
+        torch.cuda.synchronize()
+        for _ in range(2):
+            pb.check(L.pb_gemm_fp16x3_packed(h.handle, N, N, N, 1.0, ah.data_ptr(), al.data_ptr(), fac.data_ptr(),
+                                             N, bh.data_ptr(), bl.data_ptr(), fac.data_ptr() + 4, N, 0.0,
+                                             C.data_ptr(), N))
+            torch.cuda.synchronize()
+        log("pmc probe done")
+        return
     pb.check(L.pb_pack_bf16x3(h.handle, 0, b"n", N, N, A.data_ptr(), N, ah.data_ptr(), al.data_ptr(), N))
     pb.check(L.pb_pack_bf16x3(h.handle, 1, b"n", N, N, B.data_ptr(), N, bh.data_ptr(), bl.data_ptr(), N))
     torch.cuda.synchronize()
@@ -179,6 +199,7 @@ def precision_study(torch, pb, h, timer, N, kind):
     C = torch.empty(N * N, device="cuda")
     for name, prec in (("fp32_mfma", pb.gemm_precision_t.fp32_mfma),
                        ("fp32_simt", pb.gemm_precision_t.fp32_simt),
+                       ("fp16x3", pb.gemm_precision_t.fp16x3),
                        ("bf16x3", pb.gemm_precision_t.bf16x3),
                        ("bf16x4", pb.gemm_precision_t.bf16x4),
                        ("bf16", pb.gemm_precision_t.bf16),
@@ -218,14 +239,14 @@ def main():
     timer = Timer(torch)
     res: dict = {}
 
-    # ---------------- headline: BF16x3 SGEMM 8192^3 ----------------
+    # ---------------- headline: FP32-accurate split-precision SGEMM 8192^3 ----------------
     N = 8192
     flop = 2.0 * N ** 3
     g = torch.Generator(device="cuda").manual_seed(0)
     A = torch.rand(N * N, device="cuda", generator=g) * 2 - 1
     B = torch.rand(N * N, device="cuda", generator=g) * 2 - 1
     C = torch.zeros(N * N, device="cuda")
-    step = lambda: pb._gemm_ex(h, pb.gemm_precision_t.bf16x3, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N)
+    step = lambda: pb._gemm_ex(h, pb.gemm_precision_t.fp16x3, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -236,33 +257,42 @@ def main():
     elapsed = time.perf_counter() - t0
     ms_per_step = 1e3 * elapsed / args.steps
     value = flop / (ms_per_step * 1e-3) / 1e12
-    log(f"bf16x3 sgemm {N}^3: {ms_per_step:.3f} ms/step, {value:.1f} TFLOP/s")
+    log(f"fp16x3 sgemm {N}^3: {ms_per_step:.3f} ms/step, {value:.1f} TFLOP/s")
 
-    # plain FP32 MFMA on the same inputs
+    # plain FP32 MFMA and BF16x3 on the same inputs
     t_fp32 = timer(lambda: pb._gemm(h, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N), reps=3)
     res["sgemm_fp32_mfma_tflops"] = flop / t_fp32 / 1e9
-    res["bf16x3_speedup_vs_fp32"] = value / res["sgemm_fp32_mfma_tflops"]
-    log(f"fp32 mfma: {t_fp32:.3f} ms, {res['sgemm_fp32_mfma_tflops']:.1f} TFLOP/s, "
-        f"speedup {res['bf16x3_speedup_vs_fp32']:.2f}x")
+    res["fp16x3_speedup_vs_fp32"] = value / res["sgemm_fp32_mfma_tflops"]
+    t_b3 = timer(lambda: pb._gemm_ex(h, pb.gemm_precision_t.bf16x3, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N),
+                 reps=args.steps, warm=2)
+    res["sgemm_bf16x3_tflops"] = flop / t_b3 / 1e9
+    res["bf16x3_speedup_vs_fp32"] = res["sgemm_bf16x3_tflops"] / res["sgemm_fp32_mfma_tflops"]
+    log(f"fp32 mfma: {t_fp32:.3f} ms, {res['sgemm_fp32_mfma_tflops']:.1f} TFLOP/s; speedup fp16x3 "
+        f"{res['fp16x3_speedup_vs_fp32']:.2f}x, bf16x3 {res['bf16x3_speedup_vs_fp32']:.2f}x "
+        f"({res['sgemm_bf16x3_tflops']:.1f} TFLOP/s)")
 
-    # roofline of the dominant kernel: the 3-term BF16 MFMA GEMM alone
+    # roofline of the dominant kernel: the 3-term FP16 MFMA GEMM alone
     L = pb.load()
     kp = N
     ahi = torch.empty(N * kp, dtype=torch.int16, device="cuda")
     alo, bhi, blo = torch.empty_like(ahi), torch.empty_like(ahi), torch.empty_like(ahi)
-    pb.check(L.pb_pack_bf16x3(h.handle, 0, b"n", N, N, A.data_ptr(), N, ahi.data_ptr(), alo.data_ptr(), kp))
-    pb.check(L.pb_pack_bf16x3(h.handle, 1, b"n", N, N, B.data_ptr(), N, bhi.data_ptr(), blo.data_ptr(), kp))
-    kern = lambda: pb.check(L.pb_gemm_bf16x3_packed(h.handle, N, N, N, 1.0, ahi.data_ptr(), alo.data_ptr(), kp,
-                                                    bhi.data_ptr(), blo.data_ptr(), kp, 0.0, C.data_ptr(), N))
+    fac = torch.empty(2, device="cuda")
+    pb.check(L.pb_pack_fp16x3(h.handle, 0, b"n", N, N, A.data_ptr(), N, ahi.data_ptr(), alo.data_ptr(), kp,
+                              fac.data_ptr()))
+    pb.check(L.pb_pack_fp16x3(h.handle, 1, b"n", N, N, B.data_ptr(), N, bhi.data_ptr(), blo.data_ptr(), kp,
+                              fac.data_ptr() + 4))
+    kern = lambda: pb.check(L.pb_gemm_fp16x3_packed(h.handle, N, N, N, 1.0, ahi.data_ptr(), alo.data_ptr(),
+                                                    fac.data_ptr(), kp, bhi.data_ptr(), blo.data_ptr(),
+                                                    fac.data_ptr() + 4, kp, 0.0, C.data_ptr(), N))
     t_kern = timer(kern, reps=10, warm=2)
     mfma_flop = 3 * flop
     achieved = mfma_flop / (t_kern * 1e-3) / 1e12
-    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["bf16"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK["bf16"], "traffic": committed_traffic(),
-                "kernel": "mt_kernel<16,3>: 256x256 tile, BF16x3 (3 BF16 MFMA terms fused per K step)",
+    roofline = {"bound": "mfma", "achieved": achieved, "peak": PEAK["fp16"], "unit": "TFLOP/s",
+                "frac": achieved / PEAK["fp16"], "traffic": committed_traffic(),
+                "kernel": "mt_kernel<16,3,2,true>: 256x256 tile, FP16x3 (3 FP16 MFMA terms fused per K step)",
                 "kernel_ms": t_kern, "flop_per_launch": mfma_flop}
-    log(f"bf16 gemm kernel alone: {t_kern:.3f} ms, {achieved:.0f} TFLOP/s of BF16 MFMA "
-        f"({100 * achieved / PEAK['bf16']:.1f}% of peak)")
+    log(f"fp16 gemm kernel alone: {t_kern:.3f} ms, {achieved:.0f} TFLOP/s of FP16 MFMA "
+        f"({100 * achieved / PEAK['fp16']:.1f}% of peak)")
     del ahi, alo, bhi, blo
 
     if not args.quick:
@@ -274,7 +304,7 @@ def main():
                     f"normwise {e['normwise']:.3e}")
         # max componentwise error of each split format relative to plain FP32 (<= 1: no worse)
         res["max_rel_vs_fp32"] = {kind: {name: d[name]["max_rel"] / d["fp32_mfma"]["max_rel"]
-                                         for name in ("bf16x3", "bf16x4")}
+                                         for name in ("fp16x3", "bf16x3", "bf16x4")}
                                   for kind, d in res["precision_8192"].items()}
         # other variants' throughput on the headline inputs
         for name, prec in (("bf16x4", pb.gemm_precision_t.bf16x4), ("bf16", pb.gemm_precision_t.bf16),
@@ -395,9 +425,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "bf16x3 (fp32 in/out, bf16 MFMA, fp32 accumulate)",
+        "dtype": "fp16x3 (fp32 in/out, fp16 MFMA, fp32 accumulate)",
         "data": "synthetic (U[-1,1], seeded)",
-        "config": {"workload": "SGEMM 8192x8192x8192 col-major 'n','n' alpha 1 beta 0 via BF16x3 split precision",
+        "config": {"workload": "SGEMM 8192x8192x8192 col-major 'n','n' alpha 1 beta 0 via three-term split "
+                               "precision (FP16x3: the split whose max error vs FP64 is at or below plain "
+                               "FP32's; BF16x3 in extra)",
                    "M": N, "N": N, "K": N, "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -67,7 +67,13 @@ typedef enum pb_gemm_precision_t {
   PB_GEMM_BF16X3 = 2,    /* split precision: hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulate */
   PB_GEMM_BF16 = 3,      /* inputs rounded to BF16, BF16 MFMA, FP32 accumulate */
   PB_GEMM_FP8_E4M3 = 4,  /* inputs scaled by per-tensor amax to OCP e4m3, FP8 MFMA, FP32 accumulate */
-  PB_GEMM_BF16X4 = 5     /* BF16X3 plus the lo*lo term: smaller representation error */
+  PB_GEMM_BF16X4 = 5,    /* BF16X3 plus the lo*lo term: smaller representation error */
+  PB_GEMM_FP16X3 = 6     /* split precision in IEEE half: each operand scaled by a power of two
+                          * (amax over its finite elements to [2^14, 2^15)), then hi = f16(x),
+                          * lo = f16(x - hi) (22 significand bits); hi*hi + hi*lo + lo*hi on
+                          * FP16 MFMA, FP32 accumulate, unscaled in the epilogue.  Elements
+                          * below 2^-28 of their operand's amax lose significand bits (FP16
+                          * subnormals); amax must lie in [2^-59, 2^78] */
 } pb_gemm_precision_t;
 
 typedef struct pb_handle_s* pb_handle_t;
@@ -190,6 +196,22 @@ pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k
                                   const uint16_t* a_hi, const uint16_t* a_lo, int64_t ldpa,
                                   const uint16_t* b_hi, const uint16_t* b_lo, int64_t ldpb,
                                   float beta, float* c, int64_t ldc);
+
+/* The same pair for PB_GEMM_FP16X3 (no reference counterpart): op(X) is
+ * scaled by 2^s (amax of its finite elements to [2^14, 2^15), s clamped to
+ * [-63, 74]) and written as IEEE half hi = f16(x 2^s), lo = f16(x 2^s - hi)
+ * in the packed layout above; *factor (device memory) receives 2^-s. */
+pb_status_t pb_pack_fp16x3(pb_handle_t h, int operand, char trans, int64_t rows, int64_t k,
+                           const float* x, int64_t ldx, uint16_t* hi, uint16_t* lo,
+                           int64_t ldp, float* factor);
+/* C = alpha * fa * fb * (A_hi B_lo + A_lo B_hi + A_hi B_hi) + beta * C on
+ * operands from pb_pack_fp16x3, fa and fb read from the device factors it
+ * wrote. */
+pb_status_t pb_gemm_fp16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k, float alpha,
+                                  const uint16_t* a_hi, const uint16_t* a_lo, const float* a_factor,
+                                  int64_t ldpa, const uint16_t* b_hi, const uint16_t* b_lo,
+                                  const float* b_factor, int64_t ldpb, float beta, float* c,
+                                  int64_t ldc);
 
 /* BF16-in (raw uint16 bit patterns), FP32-accumulate, FP32-out GEMM. */
 pb_status_t pb_gemm_bf16(pb_handle_t h, char transa, char transb, int64_t m,

@@ -391,7 +391,8 @@ enum class gemm_precision_t : int {
   bf16x3 = PB_GEMM_BF16X3,
   bf16 = PB_GEMM_BF16,
   fp8_e4m3 = PB_GEMM_FP8_E4M3,
-  bf16x4 = PB_GEMM_BF16X4
+  bf16x4 = PB_GEMM_BF16X4,
+  fp16x3 = PB_GEMM_FP16X3
 };
 
 inline event_t _gemm_ex(SB_Handle& h, gemm_precision_t precision, char transa, char transb,

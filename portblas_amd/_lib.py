@@ -50,6 +50,8 @@ SIGNATURES: dict[str, list] = {
                      _i64, _i64, _i64],
     "pb_pack_bf16x3": [_p, _i32, _ch, _i64, _i64, _p, _i64, _p, _p, _i64],
     "pb_gemm_bf16x3_packed": [_p, _i64, _i64, _i64, _f32, _p, _p, _i64, _p, _p, _i64, _f32, _p, _i64],
+    "pb_pack_fp16x3": [_p, _i32, _ch, _i64, _i64, _p, _i64, _p, _p, _i64, _p],
+    "pb_gemm_fp16x3_packed": [_p, _i64, _i64, _i64, _f32, _p, _p, _p, _i64, _p, _p, _p, _i64, _f32, _p, _i64],
     "pb_gemm_grouped_plan_create": [_p, _i32, _p, _i64, ctypes.POINTER(_p)],
     "pb_gemm_grouped_plan_run": [_p, _p],
     "pb_gemm_grouped_plan_destroy": [_p],

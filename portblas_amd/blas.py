@@ -43,6 +43,7 @@ class gemm_precision_t(enum.IntEnum):
     bf16 = 3
     fp8_e4m3 = 4
     bf16x4 = 5
+    fp16x3 = 6
 
 
 # Operators of extension::_reduction (src/operations/blas_operators.hpp).

@@ -10,6 +10,10 @@
 //   BF16:    hi only
 //   FP8:     q = e4m3(x * 448 / amax(X)); the epilogue multiplies by
 //            amax(A) amax(B) / 448^2 (read on the device, no host round trip)
+//   FP16x3:  x_s = x * 2^s(X) with amax(X) 2^s(X) in [2^14, 2^15) (amax over
+//            the finite elements), hi = f16(x_s), lo = f16(x_s - hi);
+//            C = 2^-(s(A) + s(B)) (A_hi B_lo + A_lo B_hi + A_hi B_hi), each
+//            operand kept to 22 significand bits (BF16x3: 16)
 // The GEMM then runs on the packed operands (every operand K-contiguous and
 // 16-byte aligned): the 256 x 256 fused-term kernel (gemm_mt.hpp) when the
 // problem fills the GPU with such tiles, else the MFMA tile engine.
@@ -41,6 +45,15 @@ pb_status_t sgemm_alpha_zero(pb_handle_t h, int64_t M, int64_t N, float beta, fl
 constexpr int kAmaxBlock = 256;
 constexpr int kAmaxMaxParts = 2048;  // partial slots per operand (grid cap)
 
+// |x| as the amax passes count it: FIN (the FP16 split's scale) skips
+// infinities, which the scaled parts carry through unchanged.
+template <bool FIN>
+__device__ __forceinline__ float amag(float x) {
+  const float a = fabsf(x);
+  if constexpr (FIN) return a < __builtin_huge_valf() ? a : 0.f;
+  return a;
+}
+
 // Largest value over the workgroup, written by thread 0 to part[blockIdx.x].
 __device__ __forceinline__ void amax_block_store(float best, float* part) {
   __shared__ float red[kAmaxBlock / 64];
@@ -57,6 +70,7 @@ __device__ __forceinline__ void amax_block_store(float best, float* part) {
 }
 
 // amax over the logical rows x K elements of op(X) (all batches), any strides.
+template <bool FIN>
 __global__ __launch_bounds__(kAmaxBlock) void amax_kernel(const float* __restrict__ x, int64_t rs, int64_t ks,
                                                           int64_t rows, int64_t K, int64_t stride, int64_t batch,
                                                           float* part) {
@@ -68,13 +82,14 @@ __global__ __launch_bounds__(kAmaxBlock) void amax_kernel(const float* __restric
     const int64_t b = i / per, e = i % per;
     const int64_t r = rs == 1 ? e % rows : e / K;
     const int64_t k = rs == 1 ? e / rows : e % K;
-    best = fmaxf(best, fabsf(x[b * stride + r * rs + k * ks]));
+    best = fmaxf(best, amag<FIN>(x[b * stride + r * rs + k * ks]));
   }
   amax_block_store(best, part);
 }
 
 // amax of a dense block of n floats (an operand stored without padding):
 // 16-byte non-temporal loads, 8 in flight per lane.
+template <bool FIN>
 __global__ __launch_bounds__(kAmaxBlock) void amax_flat_kernel(const float* __restrict__ x, int64_t n,
                                                                float* part) {
   typedef float f4 __attribute__((ext_vector_type(4)));
@@ -90,31 +105,31 @@ __global__ __launch_bounds__(kAmaxBlock) void amax_flat_kernel(const float* __re
       v[u] = base + u * kAmaxBlock < nvec ? __builtin_nontemporal_load(xv + base + u * kAmaxBlock) : f4{0, 0, 0, 0};
 #pragma unroll
     for (int u = 0; u < U; ++u)
-      best = fmaxf(best, fmaxf(fmaxf(fabsf(v[u].x), fabsf(v[u].y)), fmaxf(fabsf(v[u].z), fabsf(v[u].w))));
+      best = fmaxf(best, fmaxf(fmaxf(amag<FIN>(v[u].x), amag<FIN>(v[u].y)),
+                               fmaxf(amag<FIN>(v[u].z), amag<FIN>(v[u].w))));
   }
-  if (blockIdx.x == 0 && threadIdx.x < n - nvec * 4) best = fmaxf(best, fabsf(x[nvec * 4 + threadIdx.x]));
+  if (blockIdx.x == 0 && threadIdx.x < n - nvec * 4) best = fmaxf(best, amag<FIN>(x[nvec * 4 + threadIdx.x]));
   amax_block_store(best, part);
 }
 
 // amax over `nlines` lines (per batch) of `len` contiguous floats, line j of
 // batch b at x + b * stride + j * pitch: one workgroup per line at a time,
 // coalesced along the line, no per-element index arithmetic.
+template <bool FIN>
 __global__ __launch_bounds__(kAmaxBlock) void amax_lines_kernel(const float* __restrict__ x, int64_t len,
                                                                 int64_t pitch, int64_t nlines, int64_t stride,
                                                                 int64_t batch, float* part) {
   float best = 0.f;
   for (int64_t l = blockIdx.x; l < nlines * batch; l += gridDim.x) {
     const float* line = x + (l / nlines) * stride + (l % nlines) * pitch;
-    for (int64_t i = threadIdx.x; i < len; i += kAmaxBlock) best = fmaxf(best, fabsf(line[i]));
+    for (int64_t i = threadIdx.x; i < len; i += kAmaxBlock) best = fmaxf(best, amag<FIN>(line[i]));
   }
   amax_block_store(best, part);
 }
 
-// Folds the two operands' partial maxima (na and nb slots) and derives the
-// per-tensor scales: scales[0] = 448 / amax(A), scales[1] = 448 / amax(B)
-// (1 when the amax is 0 or not finite), scales[2] = 1 / (scales[0] scales[1]).
-__global__ __launch_bounds__(kAmaxBlock) void fp8_scales_kernel(const float* pa, int na, const float* pb, int nb,
-                                                                float* scales) {
+// Folds each operand's partial maxima (na and nb slots) into amax[0..1]
+// (thread 0 of the workgroup).
+__device__ __forceinline__ void fold_amax(const float* pa, int na, const float* pb, int nb, float* amax) {
   __shared__ float red[2][kAmaxBlock / 64];
   float a = 0.f, b = 0.f;
   for (int i = threadIdx.x; i < na; i += kAmaxBlock) a = fmaxf(a, pa[i]);
@@ -135,16 +150,59 @@ __global__ __launch_bounds__(kAmaxBlock) void fp8_scales_kernel(const float* pa,
     red[0][0] = fmaxf(red[0][0], red[0][w]);
     red[1][0] = fmaxf(red[1][0], red[1][w]);
   }
-  a = red[0][0];
-  b = red[1][0];
-  const float sa = (a > 0.f && a < 3.0e38f) ? 448.0f / a : 1.0f;
-  const float sb = (b > 0.f && b < 3.0e38f) ? 448.0f / b : 1.0f;
+  amax[0] = red[0][0];
+  amax[1] = red[1][0];
+}
+
+// FP8 scales: scales[0] = 448 / amax(A), scales[1] = 448 / amax(B) (1 when
+// the amax is 0 or not finite), scales[2] = 1 / (scales[0] scales[1]).
+__global__ __launch_bounds__(kAmaxBlock) void fp8_scales_kernel(const float* pa, int na, const float* pb, int nb,
+                                                                float* scales) {
+  float am[2];
+  fold_amax(pa, na, pb, nb, am);
+  if (threadIdx.x != 0) return;
+  const float sa = (am[0] > 0.f && am[0] < 3.0e38f) ? 448.0f / am[0] : 1.0f;
+  const float sb = (am[1] > 0.f && am[1] < 3.0e38f) ? 448.0f / am[1] : 1.0f;
   scales[0] = sa;
   scales[1] = sb;
   scales[2] = (1.0f / sa) * (1.0f / sb);
 }
 
-enum PackMode { kPackBf16 = 0, kPackBf16Split = 1, kPackFp8 = 2 };
+// Power-of-two scale exponent of one FP16x3 operand: amax * 2^s in
+// [2^14, 2^15), so |hi| <= 2^15 stays finite and elements down to 2^-28
+// amax keep normal hi and lo parts.  Clamped to [-63, 74]: the unscaling
+// factor 2^-(s(A) + s(B)) then stays a finite, exactly representable float.
+__device__ __forceinline__ int f16_scale_exp(float amax) {
+  if (!(amax > 0.f)) return 0;  // all zero (or no finite element)
+  int e;
+  frexpf(amax, &e);  // amax = f 2^e, f in [0.5, 1)
+  int s = 15 - e;
+  return s < -63 ? -63 : (s > 74 ? 74 : s);
+}
+
+// FP16x3 scales: scales[0] = 2^s(A), scales[1] = 2^s(B),
+// scales[2] = 2^-(s(A) + s(B)), all exact.
+// With `factor` (one operand, nb == 0), also factor[0] = 2^-s(A), the
+// operand's share of the unscaling.
+__global__ __launch_bounds__(kAmaxBlock) void f16_scales_kernel(const float* pa, int na, const float* pb, int nb,
+                                                                float* scales, float* factor) {
+  float am[2];
+  fold_amax(pa, na, pb, nb, am);
+  if (threadIdx.x != 0) return;
+  const int ea = f16_scale_exp(am[0]), eb = f16_scale_exp(am[1]);
+  scales[0] = ldexpf(1.0f, ea);
+  scales[1] = ldexpf(1.0f, eb);
+  scales[2] = ldexpf(1.0f, -(ea + eb));
+  if (factor) factor[0] = ldexpf(1.0f, -ea);
+}
+
+enum PackMode { kPackBf16 = 0, kPackBf16Split = 1, kPackFp8 = 2, kPackF16Split = 3 };
+
+// Modes that multiply by a per-tensor scale (scales[scale_idx]) first.
+constexpr bool scaled_mode(int mode) { return mode == kPackFp8 || mode == kPackF16Split; }
+
+// IEEE binary16 bits of x, rounded to nearest even (subnormals kept).
+__device__ __forceinline__ uint16_t f16_bits(_Float16 x) { return __builtin_bit_cast(uint16_t, x); }
 
 // Index of packed element (r, k) of batch b.  slab == 0: row-major
 // [batch][rows][kp]; slab > 0: K-slab-major [batch][kp / slab][rows][slab],
@@ -178,7 +236,7 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
     tile[rr][kk] = (r < rows && k < K) ? xb[r * rs + k * ks] : 0.f;
   }
   __syncthreads();
-  const float s = (MODE == kPackFp8) ? scales[scale_idx] : 1.0f;
+  const float s = scaled_mode(MODE) ? scales[scale_idx] : 1.0f;
   for (int e = threadIdx.x; e < 64 * 64; e += 256) {
     const int rr = e / 64, kk = e % 64;
     const int64_t r = r0 + rr, k = k0 + kk;
@@ -187,6 +245,11 @@ __global__ __launch_bounds__(256) void pack_kernel(const float* __restrict__ x, 
     const float v = tile[rr][kk];
     if constexpr (MODE == kPackFp8) {
       static_cast<uint8_t*>(dst_hi)[o] = pblowp::f32_to_e4m3(v * s);
+    } else if constexpr (MODE == kPackF16Split) {
+      const float vs = v * s;
+      const _Float16 hi = _Float16(vs);
+      static_cast<uint16_t*>(dst_hi)[o] = f16_bits(hi);
+      static_cast<uint16_t*>(dst_lo)[o] = f16_bits(_Float16(vs - float(hi)));
     } else {
       const uint16_t hi = pblowp::f32_to_bf16(v);
       static_cast<uint16_t*>(dst_hi)[o] = hi;
@@ -201,6 +264,10 @@ template <int MODE>
 __device__ __forceinline__ void pack_convert(float v, uint16_t& hi, uint16_t& lo, uint8_t& q) {
   if constexpr (MODE == kPackFp8) {
     q = pblowp::f32_to_e4m3(v);
+  } else if constexpr (MODE == kPackF16Split) {
+    const _Float16 h = _Float16(v);
+    hi = f16_bits(h);
+    lo = f16_bits(_Float16(v - float(h)));  // v - h is exact in FP32
   } else {
     hi = pblowp::f32_to_bf16(v);
     if constexpr (MODE == kPackBf16Split) lo = pblowp::f32_to_bf16(v - pblowp::bf16_to_f32(hi));
@@ -261,7 +328,7 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
     }
   }
   __syncthreads();
-  const float s = (MODE == kPackFp8) ? scales[scale_idx] : 1.0f;
+  const float s = scaled_mode(MODE) ? scales[scale_idx] : 1.0f;
   const int rr = threadIdx.x % 64, kq = 16 * (threadIdx.x / 64);
   const int64_t r = r0 + rr, k = k0 + kq;
   if (r >= rows || k >= kp) return;
@@ -282,7 +349,7 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
     uint4* dh = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_hi) + o);
     dh[0] = w[0];
     dh[1] = w[1];
-    if constexpr (MODE == kPackBf16Split) {
+    if constexpr (MODE == kPackBf16Split || MODE == kPackF16Split) {
       __builtin_memcpy(w, l, 32);
       uint4* dl = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_lo) + o);
       dl[0] = w[0];
@@ -375,6 +442,50 @@ int default_config(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch
 int selector_choose(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64_t K,
                     int64_t batch);
 
+// amax of op(X) (rows x K, element (r, k) at x[r*rs + k*ks], batch b at +
+// b*stride) into one partial per workgroup; returns the partial count.
+// Dense operands stream as one block; others go line by line along their
+// contiguous dimension.
+template <bool FIN>
+int launch_amax(pb_handle_t h, const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t K, int64_t stride,
+                int64_t batch, float* part) {
+  const int64_t per = rows * K;
+  const bool dense = ((rs == 1 && ks == rows) || (ks == 1 && rs == K)) && (batch == 1 || stride == per);
+  int64_t grid;
+  if (dense && aligned16(x)) {
+    grid = std::min<int64_t>(pb::ceil_div(per * batch, 4 * 8 * kAmaxBlock), 8 * h->num_cus);
+    grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
+    hipLaunchKernelGGL(amax_flat_kernel<FIN>, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x,
+                       per * batch, part);
+  } else if (rs == 1 || ks == 1) {
+    const int64_t len = rs == 1 ? rows : K, nl = rs == 1 ? K : rows, pitch = rs == 1 ? ks : rs;
+    grid = std::min<int64_t>(nl * batch, 8 * h->num_cus);
+    grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
+    hipLaunchKernelGGL(amax_lines_kernel<FIN>, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x, len,
+                       pitch, nl, stride, batch, part);
+  } else {
+    grid = std::min<int64_t>(pb::ceil_div(per * batch, kAmaxBlock), 4 * h->num_cus);
+    grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
+    hipLaunchKernelGGL(amax_kernel<FIN>, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x, rs, ks, rows,
+                       K, stride, batch, part);
+  }
+  return int(grid);
+}
+
+// Tile plan of the FP16 tile-engine GEMMs: 128 x 128 tiles when they give
+// every CU two workgroups, else 64 x 64 tiles with K split until they do.
+struct F16Plan { bool big = false; int splits = 1; };
+
+inline F16Plan plan_f16(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const int64_t cus = h->num_cus;
+  const int64_t t128 = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
+  const int64_t t64 = pb::ceil_div(M, 64) * pb::ceil_div(N, 64) * batch;
+  F16Plan pl;
+  if (t128 >= 2 * cus) { pl.big = true; return pl; }
+  while (t64 * pl.splits < 2 * cus && K / (pl.splits * 2) >= 256 && pl.splits < 16) pl.splits *= 2;
+  return pl;
+}
+
 // ---------------------------------------------------------------------------
 // Runners
 // ---------------------------------------------------------------------------
@@ -413,9 +524,10 @@ bool mt_eligible(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch) 
 pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall<float>& g) {
   const int64_t M = g.M, N = g.N, K = g.K, batch = g.batch;
   const bool fp8 = prec == PB_GEMM_FP8_E4M3;
-  const bool split = prec == PB_GEMM_BF16X3 || prec == PB_GEMM_BF16X4;
+  const bool f16s = prec == PB_GEMM_FP16X3;
+  const bool split = prec == PB_GEMM_BF16X3 || prec == PB_GEMM_BF16X4 || f16s;
   const bool use_mt = mt_eligible(h, M, N, K, batch);
-  // The 256 x 256 kernel's K step: 16 (BF16 hi/lo), 32 (BF16) or 64 (FP8)
+  // The 256 x 256 kernel's K step: 16 (BF16 / FP16 hi/lo), 32 (BF16) or 64 (FP8)
   // elements, 32 or 64 bytes per row.
   const int slab = use_mt ? (split ? 16 : fp8 ? 64 : 32) : 0;
   // Packed pitch: the 256 x 256 kernel's ring alignment (4 K steps), else 16
@@ -429,7 +541,11 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   // workspace is taken: packed operands and partials share one allocation.
   int id = -1, fp8_splits = 1;
   size_t part_bytes = 0;
-  if (fp8 && !use_mt) {
+  F16Plan f16p{};
+  if (f16s && !use_mt) {
+    f16p = plan_f16(h, M, N, K, batch);
+    part_bytes = f16p.big ? 0 : splitk_bytes<fp16_t, B64>(M, N, K, batch, f16p.splits);
+  } else if (fp8 && !use_mt) {
     const int64_t t = pb::ceil_div(M, 128) * pb::ceil_div(N, 128) * batch;
     while (t * fp8_splits < 2 * h->num_cus && K / (fp8_splits * 2) >= 512 && fp8_splits < 16)
       fp8_splits *= 2;
@@ -454,37 +570,23 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   // Operand strides of op(A) (rows M) and op(B) (rows N), as for the engine.
   const int64_t a_rs = g.ta == 'n' ? 1 : g.lda, a_ks = g.ta == 'n' ? g.lda : 1;
   const int64_t b_rs = g.tb == 'n' ? g.ldb : 1, b_ks = g.tb == 'n' ? 1 : g.ldb;
-  if (fp8) {
+  if (fp8 || f16s) {
     // amax of each operand over all batches (one partial per workgroup),
     // then the scales (device side).  Dense operands stream as one block;
-    // others go line by line along their contiguous dimension.
+    // others go line by line along their contiguous dimension.  The FP16
+    // split's amax counts finite elements only.
     auto amax = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, float* part) -> int {
-      const int64_t per = rows * K;
-      const bool dense = ((rs == 1 && ks == rows) || (ks == 1 && rs == K)) && (batch == 1 || stride == per);
-      int64_t grid;
-      if (dense && aligned16(x)) {
-        grid = std::min<int64_t>(pb::ceil_div(per * batch, 4 * 8 * kAmaxBlock), 8 * h->num_cus);
-        grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
-        hipLaunchKernelGGL(amax_flat_kernel, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x,
-                           per * batch, part);
-      } else if (rs == 1 || ks == 1) {
-        const int64_t len = rs == 1 ? rows : K, nl = rs == 1 ? K : rows, pitch = rs == 1 ? ks : rs;
-        grid = std::min<int64_t>(nl * batch, 8 * h->num_cus);
-        grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
-        hipLaunchKernelGGL(amax_lines_kernel, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream,
-                           x, len, pitch, nl, stride, batch, part);
-      } else {
-        grid = std::min<int64_t>(pb::ceil_div(per * batch, kAmaxBlock), 4 * h->num_cus);
-        grid = std::max<int64_t>(std::min<int64_t>(grid, kAmaxMaxParts), 1);
-        hipLaunchKernelGGL(amax_kernel, dim3(unsigned(grid)), dim3(kAmaxBlock), 0, h->stream, x, rs,
-                           ks, rows, K, stride, batch, part);
-      }
-      return int(grid);
+      return f16s ? launch_amax<true>(h, x, rs, ks, rows, K, stride, batch, part)
+                  : launch_amax<false>(h, x, rs, ks, rows, K, stride, batch, part);
     };
     const int na = amax(g.a, a_rs, a_ks, M, g.sa, amax_part);
     const int nb = amax(g.b, b_rs, b_ks, N, g.sb, amax_part + kAmaxMaxParts);
-    hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
-                       amax_part + kAmaxMaxParts, nb, scales);
+    if (f16s)
+      hipLaunchKernelGGL(f16_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
+                         amax_part + kAmaxMaxParts, nb, scales, static_cast<float*>(nullptr));
+    else
+      hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
+                         amax_part + kAmaxMaxParts, nb, scales);
     PB_CHECK_LAUNCH();
   }
   PB_REQUIRE(batch <= 65535, "batch too large for the pre-pass");
@@ -507,13 +609,17 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
       hipLaunchKernelGGL((pack_vec_kernel<MODE, false>), grid, dim3(256), 0, h->stream, x,  \
                          rs, ks, rows, K, stride, kp, slab, hi, lo, scales, sidx);          \
   } while (0)
-      if (split) PB_PACKV(kPackBf16Split);
+      if (f16s) PB_PACKV(kPackF16Split);
+      else if (split) PB_PACKV(kPackBf16Split);
       else if (prec == PB_GEMM_BF16) PB_PACKV(kPackBf16);
       else PB_PACKV(kPackFp8);
 #undef PB_PACKV
       return;
     }
-    if (split)
+    if (f16s)
+      hipLaunchKernelGGL(pack_kernel<kPackF16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
+                         rows, K, stride, kp, slab, hi, lo, scales, sidx);
+    else if (split)
       hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
                          rows, K, stride, kp, slab, hi, lo, scales, sidx);
     else if (prec == PB_GEMM_BF16)
@@ -554,6 +660,30 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   const auto* al = static_cast<const uint16_t*>(a_lo);
   const auto* bh = static_cast<const uint16_t*>(b_hi);
   const auto* bl = static_cast<const uint16_t*>(b_lo);
+  if (f16s && use_mt) {
+    mt::Problem q{};
+    q.M = int(M); q.N = int(N); q.kp = int(kp);
+    q.alpha = g.alpha; q.beta = g.beta; q.alpha_dev = scales + 2;  // 2^-(s(A) + s(B))
+    q.a[0] = ah; q.a[1] = al; q.b[0] = bh; q.b[1] = bl;
+    q.a_stride = M * kp; q.b_stride = N * kp;
+    q.c = g.c; q.ldc = g.ldc; q.c_stride = g.sc; q.batch = int(batch); q.c_vec = c_vec;
+    return mt::launch<16, 3, 2, true>(h, q);
+  }
+  if (f16s) {
+    GemmProblem<fp16_t> p{};
+    p.M = M; p.N = N; p.K = K; p.alpha = g.alpha; p.beta = g.beta;
+    p.alpha_dev = scales + 2;
+    p.nterms = 3;  // small terms first, then the leading one
+    p.a[0] = ah; p.b[0] = bl;
+    p.a[1] = al; p.b[1] = bh;
+    p.a[2] = ah; p.b[2] = bh;
+    p.a_rs = kp; p.a_ks = 1; p.a_stride = M * kp;
+    p.b_rs = kp; p.b_ks = 1; p.b_stride = N * kp;
+    p.c = g.c; p.c_rs = 1; p.ldc = g.ldc; p.c_stride = g.sc; p.batch = batch; p.c_vec = c_vec;
+    p.partial = partial;
+    if (f16p.big) return launch_gemm<fp16_t, B128>(h, p, true, true, true, 1);
+    return launch_gemm<fp16_t, B64>(h, p, true, true, true, f16p.splits);
+  }
   if (use_mt) {
     mt::Problem q{};
     q.M = int(M); q.N = int(N); q.kp = int(kp);
@@ -640,13 +770,9 @@ pb_status_t gemm_f16_direct(pb_handle_t h, const GemmCall<uint16_t>& g, float al
   const bool vec = set_operands(p, g.ta, g.tb, g.lda, g.sa, g.ldb, g.sb, false);
   p.c = c; p.c_rs = 1; p.ldc = ldc; p.c_stride = sc;
   p.c_vec = aligned16(c) && ldc % 4 == 0 && (g.batch == 1 || sc % 4 == 0);
-  const int64_t cus = h->num_cus;
-  const int64_t t128 = pb::ceil_div(g.M, 128) * pb::ceil_div(g.N, 128) * g.batch;
-  const int64_t t64 = pb::ceil_div(g.M, 64) * pb::ceil_div(g.N, 64) * g.batch;
-  if (t128 >= 2 * cus) return launch_gemm<fp16_t, B128>(h, p, g.ta == 't', g.tb == 'n', vec, 1);
-  int splits = 1;
-  while (t64 * splits < 2 * cus && g.K / (splits * 2) >= 256 && splits < 16) splits *= 2;
-  return launch_gemm<fp16_t, B64>(h, p, g.ta == 't', g.tb == 'n', vec, splits);
+  const F16Plan pl = plan_f16(h, g.M, g.N, g.K, g.batch);
+  if (pl.big) return launch_gemm<fp16_t, B128>(h, p, g.ta == 't', g.tb == 'n', vec, 1);
+  return launch_gemm<fp16_t, B64>(h, p, g.ta == 't', g.tb == 'n', vec, pl.splits);
 }
 
 // C (half) <- half(T + beta * C) with T already alpha * op(A) op(B) in FP32
@@ -680,7 +806,7 @@ pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t prec, char ta, char tb
   pb::clear_error();
   PB_TRY(pb::enter(h));
   if (prec != PB_GEMM_BF16X3 && prec != PB_GEMM_BF16X4 && prec != PB_GEMM_BF16 &&
-      prec != PB_GEMM_FP8_E4M3)
+      prec != PB_GEMM_FP8_E4M3 && prec != PB_GEMM_FP16X3)
     return pb::set_error(PB_STATUS_INVALID_VALUE, "invalid precision %d", int(prec));
   GemmCall<float> g;
   bool done = false;
@@ -753,6 +879,74 @@ pb_status_t pb_gemm_bf16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k
   q.a[0] = a_hi; q.a[1] = a_lo; q.b[0] = b_hi; q.b[1] = b_lo;
   q.c = c; q.ldc = ldc; q.batch = 1; q.c_vec = aligned16(c) && ldc % 4 == 0;
   return mt::launch<16, 3>(h, q);
+}
+
+pb_status_t pb_pack_fp16x3(pb_handle_t h, int operand, char trans, int64_t rows, int64_t k,
+                           const float* x, int64_t ldx, uint16_t* hi, uint16_t* lo, int64_t ldp,
+                           float* factor) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  const char t = pb::norm_trans(trans);
+  PB_REQUIRE(t != 0, "invalid transpose flag");
+  PB_REQUIRE(operand == 0 || operand == 1, "operand must be 0 (A) or 1 (B)");
+  PB_REQUIRE(rows >= 0 && k >= 0, "negative dimension");
+  if (rows == 0 || k == 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(x && hi && lo && factor, "null pointer");
+  PB_REQUIRE(ldp >= k && ldp % mt::kKAlign<16> == 0, "ldp must be >= k and a multiple of %d",
+             mt::kKAlign<16>);
+  int64_t rs, ks;
+  if (operand == 0) { rs = t == 'n' ? 1 : ldx; ks = t == 'n' ? ldx : 1; }
+  else { rs = t == 'n' ? ldx : 1; ks = t == 'n' ? 1 : ldx; }
+  PB_REQUIRE(ldx >= ((rs == 1) ? rows : k), "leading dimension too small");
+  // The operand's amax over its finite elements, its scale, then the split.
+  void* ws = nullptr;
+  PB_TRY(pb::workspace(h, sizeof(float) * (kAmaxMaxParts + 64), &ws));
+  float* scales = static_cast<float*>(ws);
+  float* part = scales + 64;
+  const int na = launch_amax<true>(h, x, rs, ks, rows, k, 0, 1, part);
+  hipLaunchKernelGGL(f16_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, part, na,
+                     static_cast<const float*>(nullptr), 0, scales, factor);
+  dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(ldp, 64)), 1u};
+  const bool vec = aligned16(x) && (rs == 1 ? ks : rs) % 4 == 0 && (rs == 1 || ks == 1);
+  void* dh = static_cast<void*>(hi);
+  void* dl = static_cast<void*>(lo);
+  constexpr int slab = 16;
+  if (vec && ks == 1)
+    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true>), grid, dim3(256), 0, h->stream, x,
+                       rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, static_cast<const float*>(scales), 0);
+  else if (vec)
+    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false>), grid, dim3(256), 0, h->stream,
+                       x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, static_cast<const float*>(scales), 0);
+  else
+    hipLaunchKernelGGL(pack_kernel<kPackF16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
+                       rows, k, int64_t(0), ldp, slab, dh, dl, static_cast<const float*>(scales), 0);
+  PB_CHECK_LAUNCH();
+  return PB_STATUS_SUCCESS;
+}
+
+pb_status_t pb_gemm_fp16x3_packed(pb_handle_t h, int64_t m, int64_t n, int64_t k, float alpha,
+                                  const uint16_t* a_hi, const uint16_t* a_lo, const float* a_factor,
+                                  int64_t ldpa, const uint16_t* b_hi, const uint16_t* b_lo,
+                                  const float* b_factor, int64_t ldpb, float beta, float* c,
+                                  int64_t ldc) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  PB_REQUIRE(m >= 0 && n >= 0 && k >= 0, "negative dimension");
+  if (m == 0 || n == 0) return PB_STATUS_SUCCESS;
+  PB_REQUIRE(c != nullptr && ldc >= m, "bad C");
+  PB_REQUIRE(k == 0 || (a_hi && a_lo && b_hi && b_lo && a_factor && b_factor), "null operand");
+  PB_REQUIRE(ldpa == ldpb && ldpa >= k && ldpa % mt::kKAlign<16> == 0,
+             "packed pitches must match, be >= k and a multiple of %d", mt::kKAlign<16>);
+  if (alpha == 0.f || k == 0) return sgemm_alpha_zero(h, m, n, beta, c, ldc, 0, 1);
+  PB_REQUIRE(aligned16(a_hi) && aligned16(a_lo) && aligned16(b_hi) && aligned16(b_lo),
+             "packed operands must be 16-byte aligned");
+  PB_REQUIRE(mt::fits(m, n, ldpa), "packed operands too large for 32-bit offsets");
+  mt::Problem q{};
+  q.M = int(m); q.N = int(n); q.kp = int(ldpa);
+  q.alpha = alpha; q.beta = beta; q.alpha_dev = a_factor; q.alpha_dev2 = b_factor;
+  q.a[0] = a_hi; q.a[1] = a_lo; q.b[0] = b_hi; q.b[1] = b_lo;
+  q.c = c; q.ldc = ldc; q.batch = 1; q.c_vec = aligned16(c) && ldc % 4 == 0;
+  return mt::launch<16, 3, 2, true>(h, q);
 }
 
 pb_status_t pb_gemm_bf16(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,

@@ -7,8 +7,10 @@
 // 256 for FP8) with zeros in [K, kp).  Up to two parts per operand (hi, lo);
 // TERMS products are accumulated per K step:
 //   TERMS 1: Ah Bh                           (BF16, or FP8 with ESZ 1)
-//   TERMS 3: Ah Bl + Al Bh + Ah Bh           (BF16x3)
+//   TERMS 3: Ah Bl + Al Bh + Ah Bh           (BF16x3, or FP16x3 with F16)
 //   TERMS 4: Ah Bl + Al Bh + Al Bl + Ah Bh   (BF16x4: adds the lo*lo term)
+// F16 selects IEEE binary16 parts and v_mfma_f32_32x32x16_f16 (same fragment
+// shape and rate as the BF16 instruction).
 //
 // Workgroup: 256 threads = 4 waves (2 x 2), C tile 256 x 256, wave tile
 // 128 x 128 = 4 x 4 v_mfma_f32_32x32x16_bf16 tiles (256 FP32 accumulators per
@@ -47,6 +49,7 @@ struct Problem {
   int M, N, kp;          // kp: padded K (elements), a multiple of 4 K steps
   float alpha, beta;
   const float* alpha_dev;  // optional device factor multiplied into alpha
+  const float* alpha_dev2; // optional second device factor (one per operand)
   const void* a[2];        // hi, lo parts of op(A): [batch][kp / BK][M][BK]
   const void* b[2];        // hi, lo parts of op(B): [batch][kp / BK][N][BK]
   int64_t a_stride, b_stride;  // batch strides (elements)
@@ -127,7 +130,7 @@ template <int TERMS, int ESZ> constexpr int kFragReads = (TERMS == 1 ? 8 : 16) *
 
 typedef __attribute__((address_space(3))) void* lds_ptr_t;
 
-template <int BK, int TERMS, int ESZ>
+template <int BK, int TERMS, int ESZ, bool F16>
 __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   // The body is device code only (address-space casts and LDS loads have no
   // host meaning); the host pass needs just the launch stub.
@@ -216,7 +219,8 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
       foff[g][h] = frow * C::ROW_BYTES + ((chunk ^ fswz) * 16);
     }
 
-  using FragT = typename std::conditional<ESZ == 2, bf16x8, i32x8>::type;
+  using Frag2 = typename std::conditional<F16, f16x8, bf16x8>::type;
+  using FragT = typename std::conditional<ESZ == 2, Frag2, i32x8>::type;
   FragT fa[2][NP][4], fb[2][NP][4];
   auto ldfrag = [&](const uint8_t* b, int g, FragT& f) {
     u32x4 v[FC];
@@ -234,10 +238,11 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
         ldfrag(b + (NP + t) * C::TILE_BYTES + (wn + 32 * i) * C::ROW_BYTES, g, fb[buf][t][i]);
       }
   };
-  // One MFMA: BF16 32x32x16, or FP8 e4m3 32x32x64 with unit scales
+  // One MFMA: BF16 or FP16 32x32x16, or FP8 e4m3 32x32x64 with unit scales
   // (E8M0 127 = 2^0; the per-tensor scales are folded into alpha).
   auto mfma = [&](const FragT& x, const FragT& y, f32x16 c) -> f32x16 {
-    if constexpr (ESZ == 2) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, c, 0, 0, 0);
+    if constexpr (ESZ == 2 && F16) return __builtin_amdgcn_mfma_f32_32x32x16_f16(x, y, c, 0, 0, 0);
+    else if constexpr (ESZ == 2) return __builtin_amdgcn_mfma_f32_32x32x16_bf16(x, y, c, 0, 0, 0);
     else return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(x, y, c, 0, 0, 0, 127, 0, 127);
   };
 
@@ -333,7 +338,9 @@ __global__ __launch_bounds__(256, 1) void mt_kernel(Problem p) {
   // Epilogue: C = alpha * acc + beta * C.  Lane holds column wn + 32 j +
   // lane % 32 and rows wm + 32 i + 8 (e / 4) + 4 (lane / 32) + e % 4.
   float* cb = p.c + int64_t(bidx) * p.c_stride;
-  const float alpha = p.alpha_dev ? p.alpha * *p.alpha_dev : p.alpha;
+  float alpha = p.alpha;
+  if (p.alpha_dev) alpha *= *p.alpha_dev;
+  if (p.alpha_dev2) alpha *= *p.alpha_dev2;
   const float beta = p.beta;
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
@@ -378,14 +385,14 @@ inline bool fits(int64_t M, int64_t N, int64_t kp) {
   return (M + kBM) * 64 < lim && (N + kBN) * 64 < lim;
 }
 
-template <int BK, int TERMS, int ESZ = 2>
+template <int BK, int TERMS, int ESZ = 2, bool F16 = false>
 pb_status_t launch(pb_handle_t h, Problem p) {
   p.tiles_m = int(pb::ceil_div(p.M, kBM));
   p.tiles_n = int(pb::ceil_div(p.N, kBN));
   PB_REQUIRE(p.kp % kKAlign<BK> == 0, "packed pitch must be a multiple of %d", kKAlign<BK>);
   PB_REQUIRE(p.batch >= 1 && p.batch <= 65535, "batch out of range");
   const dim3 grid{unsigned(p.tiles_m) * unsigned(p.tiles_n), unsigned(p.batch)};
-  hipLaunchKernelGGL((mt_kernel<BK, TERMS, ESZ>), grid, dim3(256), 0, h->stream, p);
+  hipLaunchKernelGGL((mt_kernel<BK, TERMS, ESZ, F16>), grid, dim3(256), 0, h->stream, p);
   PB_CHECK_LAUNCH();
   return PB_STATUS_SUCCESS;
 }

@@ -8,10 +8,16 @@ reference, with forward-error bounds derived from each format's rounding:
   BF16    |ab - hi_a hi_b| <= (2 * 2^-8 + 2^-16) |a||b|
   FP8     e4m3 after scaling by 448 / amax: relative 2^-4 plus an absolute
           2^-10 * amax / 448 below the normal range
+  FP16x3  x_s = x 2^s (amax 2^s in [2^14, 2^15)) = hi + lo + r in binary16,
+          |x_s - hi| <= 2^-11 |x_s|, |r| <= 2^-22 |x_s| while lo is normal,
+          else an absolute 2^-25; the kept products miss hi*r_b + r_a*hi +
+          lo*lo (<= 3.01 * 2^-22 |a||b|) plus 2^-25 2^-s per element
 plus FP32 accumulation, (terms * K + 2) * 2^-24 |a||b|.  Shapes cover the
 256 x 256 fused-term kernel (large M, N, ragged edges, K not a multiple of
 the K step, batches) and the tile engine (small shapes)."""
 from __future__ import annotations
+
+import math
 
 import numpy as np
 import pytest
@@ -52,9 +58,9 @@ def check_lowp(handle, torch, prec, ta, tb, m, n, k, alpha=1.0, beta=0.0, batch=
     C0 = C.clone()
     pb._gemm_ex(handle, prec, ta, tb, m, n, k, alpha, A, lda, B, ldb, beta, C, ldc, sa, sb, sc, batch)
     torch.cuda.synchronize()
-    terms = {P.bf16x3: 3, P.bf16x4: 4, P.bf16: 1, P.fp8_e4m3: 1}[prec]
+    terms = {P.bf16x3: 3, P.bf16x4: 4, P.bf16: 1, P.fp8_e4m3: 1, P.fp16x3: 3}[prec]
     rep = {P.bf16x3: 3.1 * 2.0 ** -16, P.bf16x4: 2.1 * 2.0 ** -16, P.bf16: 2.01 * 2.0 ** -8,
-           P.fp8_e4m3: 0.13}[prec]
+           P.fp8_e4m3: 0.13, P.fp16x3: 3.01 * 2.0 ** -22}[prec]
     amax_a = A.abs().max().double().item()
     amax_b = B.abs().max().double().item()
     worst = 0.0
@@ -67,8 +73,11 @@ def check_lowp(handle, torch, prec, ta, tb, m, n, k, alpha=1.0, beta=0.0, batch=
         ref = alpha * (oa @ ob) + beta * c0
         absprod = oa.abs() @ ob.abs()
         tol = (rep + 1.01 * (terms * k + 2) * U) * abs(alpha) * absprod + 2 * U * abs(beta) * c0.abs()
-        if prec == P.fp8_e4m3:
-            ea, eb = 2.0 ** -10 * amax_a / 448, 2.0 ** -10 * amax_b / 448
+        if prec in (P.fp8_e4m3, P.fp16x3):
+            if prec == P.fp8_e4m3:
+                ea, eb = 2.0 ** -10 * amax_a / 448, 2.0 ** -10 * amax_b / 448
+            else:
+                ea, eb = 2.0 ** -25 / f16_scale(amax_a), 2.0 ** -25 / f16_scale(amax_b)
             sub = 1.07 * (ea * ob.abs().sum(0, keepdim=True) + eb * oa.abs().sum(1, keepdim=True)) + k * ea * eb
             tol = tol + abs(alpha) * sub
         got = cm(C[b * sc:], m, n, ldc).double()
@@ -80,7 +89,15 @@ def check_lowp(handle, torch, prec, ta, tb, m, n, k, alpha=1.0, beta=0.0, batch=
     return worst
 
 
-PRECS = ["bf16x3", "bf16x4", "bf16", "fp8_e4m3"]
+def f16_scale(amax):
+    """The FP16x3 pre-pass scale 2^s: amax 2^s in [2^14, 2^15), s in [-63, 74]."""
+    if not amax > 0:
+        return 1.0
+    s = 15 - math.frexp(amax)[1]
+    return 2.0 ** min(max(s, -63), 74)
+
+
+PRECS = ["bf16x3", "bf16x4", "bf16", "fp8_e4m3", "fp16x3"]
 
 
 @pytest.mark.parametrize("prec", PRECS)
@@ -92,7 +109,7 @@ def test_lowp_small(handle, torch_cuda, prec, ta, tb, m, n, k):
     check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], ta, tb, m, n, k, 1.5, 0.5)
 
 
-@pytest.mark.parametrize("prec", ["bf16x3", "bf16x4", "bf16", "fp8_e4m3"])
+@pytest.mark.parametrize("prec", PRECS)
 @pytest.mark.parametrize("ta,tb,m,n,k,batch", [
     ("n", "n", 4096, 4096, 256, 1),   # whole 256 x 256 tiles
     ("t", "n", 4000, 4100, 1000, 1),  # ragged M, N edges; K tail inside the packed pitch
@@ -122,7 +139,7 @@ def test_lowp_padded_leading_dimensions(handle, torch_cuda, prec, ta, tb):
 def test_lowp_wide_range_inputs(handle, torch_cuda, dist):
     import portblas_amd as pb
 
-    for prec in ("bf16x3", "bf16x4", "fp8_e4m3"):
+    for prec in ("bf16x3", "bf16x4", "fp8_e4m3", "fp16x3"):
         check_lowp(handle, torch_cuda, pb.gemm_precision_t[prec], "n", "n", 4096, 4096, 512, dist=dist, seed=5)
 
 
@@ -280,3 +297,108 @@ def test_gemm_bf16_inputs(handle, torch_cuda, ta, tb, m, n, k):
     tol = 1.01 * (k + 2) * U * 2.0 * (oa.abs() @ ob.abs()) + 2 * U * cm(C0, m, n, m).double().abs()
     err = (cm(C, m, n, m).double() - ref).abs()
     assert (err <= tol + 1e-30).all(), (err / tol).max().item()
+
+
+@pytest.mark.parametrize("dist", ["normal", "loguniform"])
+def test_fp16x3_max_error_not_above_fp32(handle, torch_cuda, dist):
+    """The FP16 split keeps 22 significand bits per operand, so at K = 4096
+    its largest componentwise error against FP64 stays at or below the plain
+    FP32 MFMA GEMM's on the same inputs (BF16x3, with 16 bits, does not)."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    m = n = 2048
+    k = 4096
+    A = make(torch, m * k, 41, dist)
+    B = make(torch, k * n, 42, dist)
+    oa, ob = cm(A, m, k, m).double(), cm(B, k, n, k).double()
+    ref = oa @ ob
+    absprod = oa.abs() @ ob.abs()
+    worst = {}
+    for prec in ("fp32_mfma", "fp16x3"):
+        C = torch.zeros(m * n, device="cuda")
+        pb._gemm_ex(handle, pb.gemm_precision_t[prec], "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+        worst[prec] = ((cm(C, m, n, m).double() - ref).abs() / absprod).max().item()
+    assert worst["fp16x3"] <= worst["fp32_mfma"], worst
+
+
+@pytest.mark.parametrize("pos", ["first", "last"])
+def test_fp16x3_scale_over_every_workgroup_and_finite_only(handle, torch_cuda, pos):
+    """The FP16x3 scale comes from the amax over the finite elements, folded
+    over every pre-pass workgroup: a needle of 1000 (against U[-1,1]) at the
+    first or last element of a 16.8M-element operand must set it (a missed
+    partial would scale the needle past 65504 into infinity), and an
+    infinity elsewhere in the operand must not (it would flush the whole
+    operand's scale to 2^-114).  Only the infinity's row of C is non-finite."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    m = n = 64
+    k = 1 << 18
+    idx = 0 if pos == "first" else -1
+    A = make(torch, m * k, 51)
+    B = make(torch, k * n, 52)
+    A[idx] = 1000.0
+    B[idx] = -1000.0
+    A[m * (k // 2) + 5] = float("inf")  # row 5, column k / 2
+    C = torch.zeros(m * n, device="cuda")
+    pb._gemm_ex(handle, pb.gemm_precision_t.fp16x3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    torch.cuda.synchronize()
+    got = cm(C, m, n, m).double()
+    assert not torch.isfinite(got[5]).any()
+    keep = [r for r in range(m) if r != 5]
+    oa, ob = cm(A, m, k, m).double()[keep], cm(B, k, n, k).double()
+    ref = oa @ ob
+    absprod = oa.abs() @ ob.abs()
+    ea, eb = 2.0 ** -25 / f16_scale(1000.0), 2.0 ** -25 / f16_scale(1000.0)
+    tol = (3.01 * 2.0 ** -22 + 1.01 * (3 * k + 2) * U) * absprod + 1.07 * (
+        ea * ob.abs().sum(0, keepdim=True) + eb * oa.abs().sum(1, keepdim=True)) + k * ea * eb
+    err = (got[keep] - ref).abs()
+    assert torch.isfinite(err).all()
+    assert (err <= tol).all(), (err / tol).max().item()
+
+
+@pytest.mark.parametrize("m,n,k", [(4096, 4096, 512), (300, 200, 100)])
+def test_pack_once_fp16x3_matches_gemm_ex(handle, torch_cuda, m, n, k):
+    """pb_pack_fp16x3 + pb_gemm_fp16x3_packed: the factor is 2^-s with
+    amax 2^s in [2^14, 2^15), hi + lo reproduces x 2^s to 2^-22 (relative)
+    plus 2^-25 (absolute), the pad is zero, and the product matches
+    pb_gemm_ex(FP16X3) bit for bit where both run the 256 x 256 kernel."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    A = make(torch, m * k, 61, "loguniform")
+    B = make(torch, k * n, 62)
+    C1 = torch.zeros(m * n, device="cuda")
+    C2 = torch.zeros(m * n, device="cuda")
+    pb._gemm_ex(handle, pb.gemm_precision_t.fp16x3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C1, m)
+    ldp = ((k + 63) // 64) * 64
+    ah = torch.empty(m * ldp, dtype=torch.int16, device="cuda")
+    al = torch.empty_like(ah)
+    bh = torch.empty(n * ldp, dtype=torch.int16, device="cuda")
+    bl = torch.empty_like(bh)
+    fac = torch.zeros(2, device="cuda")
+    pb.check(L.pb_pack_fp16x3(handle.handle, 0, b"n", m, k, A.data_ptr(), m, ah.data_ptr(), al.data_ptr(), ldp,
+                              fac.data_ptr()))
+    pb.check(L.pb_pack_fp16x3(handle.handle, 1, b"n", n, k, B.data_ptr(), k, bh.data_ptr(), bl.data_ptr(), ldp,
+                              fac.data_ptr() + 4))
+    pb.check(L.pb_gemm_fp16x3_packed(handle.handle, m, n, k, 1.0, ah.data_ptr(), al.data_ptr(), fac.data_ptr(),
+                                     ldp, bh.data_ptr(), bl.data_ptr(), fac.data_ptr() + 4, ldp, 0.0,
+                                     C2.data_ptr(), m))
+    torch.cuda.synchronize()
+    sa, sb = f16_scale(A.abs().max().item()), f16_scale(B.abs().max().item())
+    assert fac.tolist() == [1.0 / sa, 1.0 / sb]
+    if m * n >= 4096 * 4096:
+        assert torch.equal(C1, C2)
+    else:
+        oa, ob = cm(A, m, k, m).double(), cm(B, k, n, k).double()
+        tol = (3.01 * 2.0 ** -22 + 1.01 * (3 * k + 2) * U) * (oa.abs() @ ob.abs()) + 1.07 * (
+            2.0 ** -25 / sa * ob.abs().sum(0, keepdim=True) + 2.0 ** -25 / sb * oa.abs().sum(1, keepdim=True))
+        assert ((cm(C2, m, n, m).double() - oa @ ob).abs() <= tol + 1e-30).all()
+    hi16, lo16 = _unslab(ah, m, ldp), _unslab(al, m, ldp)
+    hi = hi16[:, :k].view(torch.float16).double()
+    lo = lo16[:, :k].view(torch.float16).double()
+    xs = cm(A, m, k, m).double() * sa
+    assert ((hi + lo - xs).abs() <= 2.0 ** -22 * xs.abs() + 2.0 ** -25).all()
+    assert (hi16[:, k:] == 0).all() and (lo16[:, k:] == 0).all()
