@@ -88,6 +88,16 @@ def _kind(x: Any, default: str = "f32") -> str:
     raise TypeError(f"unsupported element type {dt}")
 
 
+def _same_real_kind(*xs: Any) -> str:
+    """'f32' or 'f64' when every operand has that element type; TypeError
+    otherwise (the reference's SYMM and TRSM take one element type)."""
+    kinds = {_kind(x) for x in xs}
+    if len(kinds) != 1 or not kinds <= {"f32", "f64"}:
+        raise TypeError(f"operands must all be float32 or all float64, got "
+                        f"{[str(getattr(x, 'dtype', type(x).__name__)) for x in xs]}")
+    return kinds.pop()
+
+
 def _t(c: str) -> bytes:
     return c.encode()[:1]
 
@@ -225,7 +235,7 @@ def _symm(sb_handle, side, uplo, M, N, alpha, a, lda, b, ldb, beta, c, ldc):
     """C <- alpha A B + beta C (side 'l') or alpha B A + beta C (side 'r'),
     A symmetric with its `uplo` triangle stored (symm_interface.hpp:36-80)."""
     L = sb_handle._L
-    fn = L.pb_dsymm if _kind(c) == "f64" else L.pb_ssymm
+    fn = L.pb_dsymm if _same_real_kind(a, b, c) == "f64" else L.pb_ssymm
     check(fn(sb_handle.handle, _t(side), _t(uplo), M, N, alpha, _addr(a), lda, _addr(b), ldb, beta,
              _addr(c), ldc))
 
@@ -234,7 +244,7 @@ def _trsm(sb_handle, side, uplo, trans, diag, M, N, alpha, a, lda, b, ldb):
     """Solves op(A) X = alpha B (side 'l') or X op(A) = alpha B (side 'r'),
     overwriting B with X (trsm_interface.hpp:36-420)."""
     L = sb_handle._L
-    fn = L.pb_dtrsm if _kind(b) == "f64" else L.pb_strsm
+    fn = L.pb_dtrsm if _same_real_kind(a, b) == "f64" else L.pb_strsm
     check(fn(sb_handle.handle, _t(side), _t(uplo), _t(trans), _t(diag), M, N, alpha, _addr(a), lda,
              _addr(b), ldb))
 

@@ -259,6 +259,177 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
   }
 }
 
+// ---------------------------------------------------------------------------
+// Ring variant of the lean kernel for problems whose M, N and K are multiples
+// of 8 with 16-byte aligned operands (every 16-byte group of 8 elements is
+// then wholly inside or wholly outside the operand).  Each workgroup walks the
+// flattened sequence of its tiles' K chunks with kRD chunks in flight in a
+// register ring, so a chunk waits for a fraction of one memory latency
+// instead of a whole one: every thread issues exactly one 16-byte load per
+// operand per chunk (lanes outside the operand read a zero block), which
+// keeps the compiler's wait counts exact, and chunks past the end of the list
+// re-read the last one (never consumed).  Two LDS buffers take alternate
+// chunks, so one barrier per chunk separates a buffer's reads from its next
+// writes.
+// ---------------------------------------------------------------------------
+constexpr int kRD = 4;  // K chunks in flight per workgroup
+
+__device__ __attribute__((aligned(16))) const uint32_t kZero16[4] = {0u, 0u, 0u, 0u};
+
+struct RingCursor {
+  const uint16_t* a;
+  const uint16_t* b;
+  int64_t a_rs, a_ks, b_rs, b_ks, M, N, K, m0, n0;
+  int i, kc, nk;  // tile (list index), chunk, chunks of the tile
+  int prob;
+};
+
+__device__ __forceinline__ void ring_set_tile(RingCursor& c, const DevProblem* probs, const int4* tiles, int i) {
+  const int4 t = tiles[i];
+  const DevProblem& P = probs[t.x];
+  c.a = static_cast<const uint16_t*>(P.a);
+  c.b = static_cast<const uint16_t*>(P.b);
+  c.a_rs = P.a_rs; c.a_ks = P.a_ks; c.b_rs = P.b_rs; c.b_ks = P.b_ks;
+  c.M = P.M; c.N = P.N; c.K = P.K;
+  c.m0 = int64_t(t.y) * kLT;
+  c.n0 = int64_t(t.z) * kLT;
+  c.i = i;
+  c.kc = 0;
+  c.nk = P.K > 0 ? int((P.K + kLK - 1) / kLK) : 1;
+  c.prob = t.x;
+}
+
+// One 16-byte load per operand for chunk c.kc of tile c.i (zero block for
+// the groups outside the operand).
+template <bool KCA, bool KCB>
+__device__ __forceinline__ void ring_load(const RingCursor& c, u32x4& va, u32x4& vb) {
+  const int e = threadIdx.x;
+  const int64_t k0 = int64_t(c.kc) * kLK;
+  const uint16_t* zero = reinterpret_cast<const uint16_t*>(kZero16);
+  {
+    const int64_t r = KCA ? c.m0 + e / kLG : c.m0 + (e % 8) * 8;
+    const int64_t k = KCA ? k0 + (e % kLG) * 8 : k0 + e / 8;
+    const uint16_t* p = (r < c.M && k < c.K) ? c.a + r * c.a_rs + k * c.a_ks : zero;
+    va = *reinterpret_cast<const u32x4*>(p);
+  }
+  {
+    const int64_t r = KCB ? c.n0 + e / kLG : c.n0 + (e % 8) * 8;
+    const int64_t k = KCB ? k0 + (e % kLG) * 8 : k0 + e / 8;
+    const uint16_t* p = (r < c.N && k < c.K) ? c.b + r * c.b_rs + k * c.b_ks : zero;
+    vb = *reinterpret_cast<const u32x4*>(p);
+  }
+}
+
+template <bool KCA, bool KCB>
+__global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __restrict__ probs,
+                                                           const int4* __restrict__ tiles,
+                                                           const int* __restrict__ starts) {
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][kLT][kLP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kLT][kLP];
+  const int begin = starts[blockIdx.x], end = starts[blockIdx.x + 1];
+  if (begin >= end) return;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
+  // Producer (kRD chunks ahead) and consumer cursors.
+  RingCursor pc, cc;
+  ring_set_tile(pc, probs, tiles, begin);
+  cc = pc;
+  auto advance_producer = [&]() {
+    if (pc.kc + 1 < pc.nk) ++pc.kc;
+    else if (pc.i + 1 < end) ring_set_tile(pc, probs, tiles, pc.i + 1);
+    // else: stay on the last chunk (re-read, never consumed)
+  };
+  u32x4 va[kRD], vb[kRD];
+#pragma unroll
+  for (int s = 0; s < kRD; ++s) {
+    ring_load<KCA, KCB>(pc, va[s], vb[s]);
+    advance_producer();
+  }
+  f32x16 acc;
+#pragma unroll
+  for (int e = 0; e < 16; ++e) acc[e] = 0.f;
+  bool more = true;
+  // One chunk: stage slot S into LDS buffer S % 2, refill the slot kRD
+  // chunks ahead, run the chunk's MFMAs, and finish the tile after its last
+  // chunk.  kRD is even, so the buffer parity follows the slot.
+  auto step = [&](auto slot) {
+    constexpr int S = decltype(slot)::value;
+    constexpr int BUF = S & 1;
+#pragma unroll
+    for (int it = 0; it < kLIt; ++it) {
+      const int e = threadIdx.x + 256 * it;
+      lean_store<KCA>(As[BUF], e, va[S]);
+      lean_store<KCB>(Bs[BUF], e, vb[S]);
+    }
+    __syncthreads();
+    ring_load<KCA, KCB>(pc, va[S], vb[S]);
+    advance_producer();
+#pragma unroll
+    for (int kk = 0; kk < kLK; kk += 16) {
+      bf16x8 a, b;
+      const int ra = wm + lane % 32, rb = wn + lane % 32, c = kk / 8 + lane / 32;
+      __builtin_memcpy(&a, &As[BUF][ra][(c ^ lean_swz(ra)) * 8], 16);
+      __builtin_memcpy(&b, &Bs[BUF][rb][(c ^ lean_swz(rb)) * 8], 16);
+      acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
+    }
+    if (cc.kc + 1 < cc.nk) {
+      ++cc.kc;
+      return;
+    }
+    // Epilogue: lane holds column wn + lane % 32, rows wm + 8 (e / 4) +
+    // 4 (lane / 32) + e % 4.
+    const DevProblem& P = probs[cc.prob];
+    const int64_t n = cc.n0 + wn + lane % 32;
+    if (n < P.N) {
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int64_t m = cc.m0 + wm + 8 * q + 4 * (lane / 32);
+        float* cp = P.c + m + n * P.ldc;
+        if (P.c_vec && m + 3 < P.M) {
+          f32x4 o;
+          if (P.beta != 0.f) {
+            const f32x4 old = *reinterpret_cast<const f32x4*>(cp);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e] + P.beta * old[e];
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e];
+          }
+          *reinterpret_cast<f32x4*>(cp) = o;
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (m + e < P.M) {
+              const float v = P.alpha * acc[4 * q + e];
+              cp[e] = (P.beta != 0.f) ? v + P.beta * cp[e] : v;
+            }
+        }
+      }
+    }
+#pragma unroll
+    for (int e = 0; e < 16; ++e) acc[e] = 0.f;
+    if (cc.i + 1 < end) ring_set_tile(cc, probs, tiles, cc.i + 1);
+    else more = false;
+  };
+  using I0 = std::integral_constant<int, 0>;
+  using I1 = std::integral_constant<int, 1>;
+  using I2 = std::integral_constant<int, 2>;
+  using I3 = std::integral_constant<int, 3>;
+  static_assert(kRD == 4, "the step sequence below is written for four slots");
+  while (more) {
+    step(I0{});
+    if (!more) break;
+    step(I1{});
+    if (!more) break;
+    step(I2{});
+    if (!more) break;
+    step(I3{});
+  }
+  // The ring's last loads (re-reads, never consumed) land before the wave
+  // ends and its registers go to another wave.
+  __builtin_amdgcn_s_waitcnt(0);
+}
+
 using G64 = TileShape<64, 64, 64, 32, 32>;
 using GF64 = TileShape<64, 64, 32, 32, 32>;
 
@@ -267,6 +438,7 @@ struct Launch {
   int count;
   int64_t tiles;
   DevProblem* dev;
+  bool ring = false;  // BF16: the ring kernel (dimensions multiples of 8, aligned)
   // BF16: the balanced static schedule (grid workgroups, tile lists).
   int grid = 0;
   const int4* tile_list = nullptr;
@@ -286,6 +458,15 @@ namespace pbgemm {
 
 void launch_lean(pb_handle_t h, const Launch& L) {
   const dim3 grid(unsigned(L.grid));
+  if (L.ring) {
+    switch (L.layout) {
+      case 3: hipLaunchKernelGGL((grouped_ring_kernel<true, true>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+      case 2: hipLaunchKernelGGL((grouped_ring_kernel<true, false>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+      case 1: hipLaunchKernelGGL((grouped_ring_kernel<false, true>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+      default: hipLaunchKernelGGL((grouped_ring_kernel<false, false>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
+    }
+    return;
+  }
   switch (L.layout) {
     case 3: hipLaunchKernelGGL((grouped_lean_kernel<true, true>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
     case 2: hipLaunchKernelGGL((grouped_lean_kernel<true, false>), grid, dim3(256), 0, h->stream, L.dev, L.tile_list, L.starts); break;
@@ -411,7 +592,9 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     delete plan;
   };
   constexpr int bm = 64, bn = 64;  // BF16: the lean kernel; FP32: the 64 x 64 tile engine
-  std::vector<DevProblem> by_layout[4];
+  // Layouts 0-3 (2 * kca + kcb); BF16 problems the ring kernel takes go to
+  // the same layout + 4.
+  std::vector<DevProblem> by_layout[8];
   for (int64_t i = 0; i < count; ++i) {
     const pb_gemm_problem_t& q = problems[i];
     const char ta = pb::norm_trans(q.transa), tb = pb::norm_trans(q.transb);
@@ -437,7 +620,8 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     d.tiles_n = int32_t(pb::ceil_div(q.n, bn));
     d.vec = aligned16(q.a) && aligned16(q.b) && q.lda % vw == 0 && q.ldb % vw == 0;
     d.c_vec = aligned16(q.c) && q.ldc % 4 == 0;
-    by_layout[2 * (ta == 't') + (tb == 'n')].push_back(d);
+    const bool ring = bf16 && d.vec && q.m % 8 == 0 && q.n % 8 == 0 && q.k % 8 == 0;
+    by_layout[2 * (ta == 't') + (tb == 'n') + (ring ? 4 : 0)].push_back(d);
   }
   auto upload = [&](const void* src, size_t bytes, void** dst) -> pb_status_t {
     hipError_t e = hipMalloc(dst, bytes);
@@ -459,6 +643,10 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     int n = 0;
     hipError_t e = hipSuccess;
     switch (layout) {
+      case 7: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_ring_kernel<true, true>, 256, 0); break;
+      case 6: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_ring_kernel<true, false>, 256, 0); break;
+      case 5: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_ring_kernel<false, true>, 256, 0); break;
+      case 4: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_ring_kernel<false, false>, 256, 0); break;
       case 3: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<true, true>, 256, 0); break;
       case 2: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<true, false>, 256, 0); break;
       case 1: e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, grouped_lean_kernel<false, true>, 256, 0); break;
@@ -470,7 +658,7 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     }
     return n;
   };
-  for (int lay = 3; lay >= 0; --lay) {
+  for (int lay = 7; lay >= 0; --lay) {
     auto& v = by_layout[lay];
     if (v.empty()) continue;
     // Largest problems first: for the FP32 work queue their tiles are handed
@@ -487,7 +675,8 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
       destroy();
       return pb::set_error(PB_STATUS_INVALID_VALUE, "grouped BF16 GEMM: more than 2^31 tiles");
     }
-    Launch L{lay, int(v.size()), t, nullptr};
+    Launch L{lay % 4, int(v.size()), t, nullptr};
+    L.ring = lay >= 4;
     void* dev = nullptr;
     PB_TRY(upload(v.data(), v.size() * sizeof(DevProblem), &dev));
     L.dev = static_cast<DevProblem*>(dev);

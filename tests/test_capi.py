@@ -86,3 +86,22 @@ def test_status_exceptions_map_like_the_reference():
     with pytest.raises(RuntimeError):
         pb.check(3)
     pb.check(0)
+
+
+def test_symm_trsm_reject_mixed_or_unsupported_element_types():
+    """_symm and _trsm take one real element type for every operand: a float64
+    A with float32 B, or bfloat16 operands, raise TypeError before any call
+    into the library (no handle or GPU needed)."""
+    import types
+
+    import torch
+
+    fake = types.SimpleNamespace(_L=None, handle=None)
+    f32, f64 = torch.zeros(4), torch.zeros(4, dtype=torch.float64)
+    bf = torch.zeros(4, dtype=torch.bfloat16)
+    with pytest.raises(TypeError):
+        pb._trsm(fake, "l", "l", "n", "n", 2, 2, 1.0, f64, 2, f32, 2)
+    with pytest.raises(TypeError):
+        pb._trsm(fake, "l", "l", "n", "n", 2, 2, 1.0, bf, 2, bf, 2)
+    with pytest.raises(TypeError):
+        pb._symm(fake, "l", "u", 2, 2, 1.0, f32, 2, f32, 2, 0.0, f64, 2)

@@ -83,6 +83,53 @@ def test_grouped_uneven_problems(handle, torch_cuda, prec, dtype_name):
     _check(bufs)
 
 
+def test_grouped_ring_kernel_shapes(handle, torch_cuda):
+    """Problems whose M, N, K are multiples of 8 with aligned operands go to
+    the ring-pipelined kernel (4 K chunks in flight per workgroup), the others
+    in the same plan to the lean kernel: both checked per problem, every
+    layout, alpha and beta rules, tile edges, K = 8 and padded leading
+    dimensions (ldc padding untouched)."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    rng = np.random.default_rng(21)
+    dims = [tuple(int(v) for v in np.maximum(8, np.round(np.exp(rng.uniform(np.log(8), np.log(700), 3)) / 8) * 8))
+            for _ in range(48)]
+    dims += [(8, 8, 8), (64, 64, 8), (72, 136, 264), (1024, 32, 1024), (8, 1000, 16), (520, 8, 40)]
+    probs, bufs = _make_problems(torch, dims, torch.bfloat16, 23, pad=8)
+    odd = [(77, 130, 41), (64, 64, 63), (9, 9, 9)]
+    probs2, bufs2 = _make_problems(torch, odd, torch.bfloat16, 24, pad=1)
+    allp = (Prob * (len(dims) + len(odd)))(*(list(probs) + list(probs2)))
+    L = pb.load()
+    pb.check(L.pb_gemm_grouped(handle.handle, int(pb.gemm_precision_t.bf16), allp, len(allp)))
+    torch.cuda.synchronize()
+    _check(bufs)
+    _check(bufs2)
+
+
+def test_grouped_benchmark_set_full_size(handle, torch_cuda):
+    """The benchmark's 512-problem set (M, N, K log-uniform in [32, 1024],
+    multiples of 8, BF16 in, FP32 out, alpha 1, beta 0), every problem checked."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    rng = np.random.default_rng(4)
+    dims = np.exp(rng.uniform(np.log(32), np.log(1024), size=(512, 3)))
+    dims = [tuple(int(v) for v in row) for row in np.maximum(8, np.round(dims / 8) * 8).astype(np.int64)]
+    g = torch.Generator(device="cuda").manual_seed(9)
+    bufs, probs = [], (Prob * len(dims))()
+    for i, (m, n, k) in enumerate(dims):
+        a = (torch.rand(m * k, generator=g, device="cuda") * 2 - 1).to(torch.bfloat16)
+        b = (torch.rand(k * n, generator=g, device="cuda") * 2 - 1).to(torch.bfloat16)
+        c = torch.full((m * n,), float("nan"), device="cuda")
+        probs[i] = Prob(b"n", b"n", m, n, k, 1.0, 0.0, a.data_ptr(), m, b.data_ptr(), k, c.data_ptr(), m)
+        bufs.append(("n", "n", m, n, k, 1.0, 0.0, a, m, b, k, c, torch.zeros_like(c), m))
+    L = pb.load()
+    pb.check(L.pb_gemm_grouped(handle.handle, int(pb.gemm_precision_t.bf16), probs, len(dims)))
+    torch.cuda.synchronize()
+    _check(bufs)
+
+
 def test_grouped_plan_reuse_and_graph_replay(handle, torch_cuda):
     """A plan runs many times; captured in a graph it replays to the eager
     result each time (the queue resets itself, no memset node)."""

@@ -55,14 +55,30 @@ def test_reference_matrix_is_diagonally_dominant_and_keeps_unused():
     assert np.all(np.abs(low).sum(axis=1) < 4.0) and np.all(np.diag(A) == 4.0)
 
 
-def _check(handle, torch, dtype, side, uplo, trans, diag, M, N, unused, seed):
+def _unit_triangle(rng, k, lda, uplo, diag, c, unused, dtype):
+    """Ones on the diagonal (stored only for diag 'n'), off-diagonal entries
+    uniform in [-c, c] in the stored triangle, `unused` elsewhere: not
+    diagonally dominant, and for c of order one badly conditioned (cond 1e4 to
+    1e8 at the sizes tested)."""
+    A = np.full((k, lda), unused, np.float64)  # A[col, row], column-major storage
+    for j in range(k):
+        for i in (range(j + 1, k) if uplo == "l" else range(j)):
+            A[j, i] = rng.uniform(-c, c)
+        A[j, j] = 1.0 if diag == "n" else unused
+    return A.reshape(-1).astype(dtype)
+
+
+def _check(handle, torch, dtype, side, uplo, trans, diag, M, N, unused, seed, offdiag=None):
     import portblas_amd as pb
 
     rng = np.random.default_rng(seed)
     k = M if side == "l" else N
     lda, ldb = 2 * k, 2 * M
-    dval = rng.uniform(1.0, 10.0)
-    a = ot.trsm_matrix(rng, k, lda, uplo, diag, dval, unused, dtype)
+    if offdiag is None:
+        dval = rng.uniform(1.0, 10.0)
+        a = ot.trsm_matrix(rng, k, lda, uplo, diag, dval, unused, dtype)
+    else:
+        a = _unit_triangle(rng, k, lda, uplo, diag, offdiag, unused, dtype)
     b = (rng.random(ldb * N) * 2 - 1).astype(dtype)
     ta, tb = torch.from_numpy(a.copy()).cuda(), torch.from_numpy(b.copy()).cuda()
     pb._trsm(handle, side, uplo, trans, diag, M, N, 2.0, ta, lda, tb, ldb)
@@ -119,6 +135,20 @@ def test_gpu_trsm_split_trailing_updates(handle, torch_cuda, side, uplo, trans, 
     update does not (n or m = 512: 256 tiles of 64 x 64 at full width, 192 at
     1536 rows) or splits less (256: 2-way at full width, 3-way at 1536 rows)."""
     _check(handle, torch_cuda, dtype, side, uplo, trans, diag, M, N, np.nan, seed=M + 3 * N)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("side,uplo,trans,diag", [("l", "l", "n", "n"), ("l", "u", "t", "u"), ("r", "u", "n", "n"),
+                                                  ("r", "l", "t", "u")])
+@pytest.mark.parametrize("k,c", [(513, 0.3), (200, 0.5), (128, 1.0)])
+@pytest.mark.parametrize("dtype", [np.float32, np.float64])
+def test_gpu_trsm_ill_conditioned(handle, torch_cuda, side, uplo, trans, diag, k, c, dtype):
+    """Unit-diagonal triangles with off-diagonal entries of order c (cond(A)
+    about 4e4, 6e4 and 5e8; the 64 x 64 diagonal blocks about 15, 130 and 1e5):
+    the explicitly inverted diagonal blocks must still give the componentwise
+    residual bound that substitution guarantees."""
+    M, N = (k, 96) if side == "l" else (96, k)
+    _check(handle, torch_cuda, dtype, side, uplo, trans, diag, M, N, np.nan, seed=k + int(10 * c), offdiag=c)
 
 
 @pytest.mark.gpu
