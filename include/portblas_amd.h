@@ -453,6 +453,37 @@ pb_status_t pb_tune_set_selector(pb_handle_t h, int n_layers, const int* layer_s
 int pb_tune_select(pb_handle_t h, char transa, char transb, int64_t m, int64_t n,
                    int64_t k, int64_t batch);
 
+/* The same for the plain FP32 / FP64 engine behind pb_sgemm / pb_dgemm and
+ * their batched forms (src/interface/blas3/backend/amd_gpu.hpp:52-288 picks
+ * these for the reference's own types).  Configurations: 128 x 128 tiles
+ * with split-K 1, 2, ..., 128 (ids 0-7), 64 x 64 tiles with the same splits
+ * (8-15), and the small-problem batch kernel (16, only for M, N <= 64 with
+ * at least one problem per CU).  dtype: 1 = FP32, 2 = FP64. */
+int pb_tune_num_plain_configs(void);
+const char* pb_tune_plain_config_name(int id);
+/* One strided-batched GEMM (default strides) with config `id` (-1: the
+ * built-in rule); alpha and beta are converted to the element type.
+ * PB_STATUS_NOT_SUPPORTED for the small-batch config on a shape it cannot run. */
+pb_status_t pb_gemm_plain_config(pb_handle_t h, int dtype, int id, char transa, char transb,
+                                 int64_t m, int64_t n, int64_t k, double alpha, const void* a,
+                                 int64_t lda, const void* b, int64_t ldb, double beta, void* c,
+                                 int64_t ldc, int64_t batch);
+/* Mean device time per call (microseconds) of config `id` (-1: the built-in
+ * rule), alpha 1, beta 0, over `reps` launches after `warmup`. */
+pb_status_t pb_tune_time_plain(pb_handle_t h, int dtype, int id, char transa, char transb,
+                               int64_t m, int64_t n, int64_t k, const void* a, int64_t lda,
+                               const void* b, int64_t ldb, void* c, int64_t ldc, int64_t batch,
+                               int warmup, int reps, float* usec_out);
+/* Install a network with pb_tune_num_plain_configs() outputs for the plain
+ * engine (features: log2 m, n, k, transa, transb, dtype, log2 batch, log2 of
+ * lda over the rows of A); n_layers = 0 returns to the built-in rule. */
+pb_status_t pb_tune_set_plain_selector(pb_handle_t h, int n_layers, const int* layer_sizes,
+                                       const float* const* weights, const float* const* biases,
+                                       const float* feat_mean, const float* feat_std);
+/* The config the installed plain network picks (-1: none installed). */
+int pb_tune_select_plain(pb_handle_t h, int dtype, char transa, char transb, int64_t m, int64_t n,
+                         int64_t k, int64_t batch, int64_t lda);
+
 /* ------------------------------------------------------------------------ */
 /* Learned-selector kernels: MLP forward/backward/update on the GPU.        */
 /* Layer l maps width d_l to d_{l+1}: Z = X W^T + b, ReLU on hidden layers, */

@@ -108,6 +108,14 @@ SIGNATURES: dict[str, list] = {
                             _f32, _p, _i64, _i64, _i64],
     "pb_tune_set_selector": [_p, _i32, _p, _p, _p, _p, _p],
     "pb_tune_select": [_p, _ch, _ch, _i64, _i64, _i64, _i64],
+    "pb_tune_num_plain_configs": [],
+    "pb_tune_plain_config_name": [_i32],
+    "pb_gemm_plain_config": [_p, _i32, _i32, _ch, _ch, _i64, _i64, _i64, _f64, _p, _i64, _p, _i64, _f64, _p,
+                             _i64, _i64],
+    "pb_tune_time_plain": [_p, _i32, _i32, _ch, _ch, _i64, _i64, _i64, _p, _i64, _p, _i64, _p, _i64, _i64,
+                           _i32, _i32, ctypes.POINTER(_f32)],
+    "pb_tune_set_plain_selector": [_p, _i32, _p, _p, _p, _p, _p],
+    "pb_tune_select_plain": [_p, _i32, _ch, _ch, _i64, _i64, _i64, _i64, _i64],
     "pb_mlp_forward": [_p, _i32, _p, _p, _p, _p, _i64],
     "pb_mlp_forward_bf16": [_p, _i32, _p, _p, _p, _p, _p, _i64],
     "pb_mlp_backward": [_p, _i32, _p, _p, _p, _p, _p, _p, _p, _p, _p, _i64],
@@ -124,6 +132,9 @@ _RESTYPES = {
     "pb_tune_num_configs": _i32,
     "pb_tune_config_name": ctypes.c_char_p,
     "pb_tune_select": _i32,
+    "pb_tune_num_plain_configs": _i32,
+    "pb_tune_plain_config_name": ctypes.c_char_p,
+    "pb_tune_select_plain": _i32,
 }
 
 _lib = None

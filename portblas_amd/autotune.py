@@ -79,7 +79,12 @@ def main(argv=None):
     ap.add_argument("--report", default=None)
     ap.add_argument("--fixture", default=None,
                     help="also save the held-out set, its timings and the built-in rule's choices (float32)")
+    ap.add_argument("--family", choices=("bf16", "plain"), default="bf16",
+                    help="bf16: the BF16-input engine (40 configs); plain: the FP32/FP64 engine behind "
+                         "pb_sgemm/pb_dgemm (17 configs, timed with the built-in rule beside them)")
     args = ap.parse_args(argv)
+    if args.family == "plain":
+        return main_plain(args)
 
     import torch
 
@@ -166,6 +171,122 @@ def main(argv=None):
         np.savez_compressed(args.fixture, X=Xte.astype(np.float32), T=Tte.astype(np.float32),
                             shapes=np.array([_enc(s) for s in te_shapes], dtype=np.int64),
                             rule_choice=chosen_rule.astype(np.int32), best_single=np.int32(best_single))
+    if args.report:
+        Path(args.report).parent.mkdir(parents=True, exist_ok=True)
+        Path(args.report).write_text(json.dumps(rep, indent=1))
+    return rep
+
+
+def _ratio_stats(t: np.ndarray, best: np.ndarray) -> dict:
+    r = t / best
+    return {"geomean_slowdown": float(np.exp(np.mean(np.log(r)))), "median_slowdown": float(np.median(r)),
+            "frac_within_5pct": float(np.mean(r <= 1.05)), "p90_slowdown": float(np.quantile(r, 0.9)),
+            "total_time_ratio": float(t.sum() / best.sum())}
+
+
+def _plain_set(tuning, tuner_box, n, seed, max_flop, reps, path, progress_every):
+    if path and Path(path).exists():
+        z = np.load(path, allow_pickle=False)
+        print(f"[tune] loaded {path}: {len(z['X'])} shapes", flush=True)
+        return z["X"], z["T"], [_dec_plain(r) for r in z["shapes"]]
+    shapes = tuning.sample_plain_shapes(n, seed=seed, max_flop=max_flop)
+    t0 = time.time()
+    X, T = tuning.collect_plain(tuner_box(), shapes, progress=progress_every, reps=reps)
+    print(f"[tune] timed {n} shapes x {T.shape[1]} configs in {time.time() - t0:.0f} s", flush=True)
+    if path:
+        Path(path).parent.mkdir(parents=True, exist_ok=True)
+        np.savez(path, X=X, T=T, shapes=np.array([_enc_plain(s) for s in shapes], dtype=np.int64))
+    return X, T, shapes
+
+
+def _enc_plain(s):
+    return [s[0], ord(s[1]), ord(s[2]), *s[3:]]
+
+
+def _dec_plain(row):
+    return (int(row[0]), chr(int(row[1])), chr(int(row[2])), *(int(v) for v in row[3:]))
+
+
+def main_plain(args):
+    """The FP32/FP64 plain engine: exhaustive timing of its 17 configurations
+    and of the built-in rule, the MLP trained on the GPU, and its held-out
+    quality against both."""
+    import torch
+
+    import portblas_amd as pb
+    from portblas_amd import tuning
+
+    h = pb.SB_Handle(selector=None, plain_selector=None)
+    nconf = tuning.num_plain_configs()
+    tuner = []
+
+    def tuner_box():
+        if not tuner:
+            tuner.append(tuning.PlainAutotuner(h))
+        return tuner[0]
+
+    t0 = time.time()
+    train_files = args.train_files or [None]
+    parts = [_plain_set(tuning, tuner_box, args.train, args.seed + i, args.max_flop, args.reps, f,
+                        max(1, args.train // 10)) for i, f in enumerate(train_files)]
+    Xtr = np.concatenate([p[0] for p in parts])
+    Ttr = np.concatenate([p[1] for p in parts])
+    Xte, Tte, te_shapes = _plain_set(tuning, tuner_box, args.heldout, args.seed + 1000, args.max_flop, args.reps,
+                                     args.heldout_file, max(1, args.heldout // 5))
+    if args.collect_only:
+        return {"train_shapes": int(len(Xtr)), "heldout_shapes": int(len(Xte))}
+    tuner.clear()
+    torch.cuda.empty_cache()
+    t_collect = time.time() - t0
+    Ytr = tuning.targets(Ttr[:, :nconf])
+    perm = np.random.default_rng(args.seed).permutation(len(Xtr))
+    n_val = int(round(args.val_frac * len(Xtr))) if args.candidates > 1 else 0
+    val, fit = perm[:n_val], perm[n_val:]
+    best_tr = Ttr[:, :nconf].min(axis=1)
+    t1 = time.time()
+    sel, best_score, val_scores = None, None, []
+    for c in range(args.candidates):
+        cand = tuning.Selector.init(sizes=(tuning.NUM_FEATURES, 64, 64, 64, nconf), seed=args.seed + c)
+        cand.train(h, Xtr[fit], Ytr[fit], mask=tuning.weights(Ytr[fit], args.sharpness), steps=args.steps,
+                   batch=1024, lr=args.lr, seed=args.seed + c, log_every=max(1, args.steps // 4))
+        if n_val:
+            ch = np.argmin(cand.predict_bf16(h, Xtr[val]), axis=1)
+            ev = _ratio_stats(tuning.plain_time_of_choice(ch, Ttr[val]), best_tr[val])
+            score = ev["geomean_slowdown"] * ev["total_time_ratio"]
+        else:
+            ev, score = None, 0.0
+        val_scores.append({"seed": args.seed + c, "validation": ev})
+        print(f"[tune] candidate {c}: validation {ev}", flush=True)
+        if sel is None or score < best_score:
+            sel, best_score = cand, score
+    t_train = time.time() - t1
+    chosen = np.argmin(sel.predict_bf16(h, Xte), axis=1)
+    best_te = Tte[:, :nconf].min(axis=1)
+    t_mlp = tuning.plain_time_of_choice(chosen, Tte)
+    t_rule = Tte[:, nconf]
+    tuning.install_plain(sel, h)
+    host = np.array([tuning.select_plain(h, *s) for s in te_shapes])
+    tuning.uninstall_plain(h)
+    rep = {
+        "family": "plain (FP32/FP64 _gemm)", "configs": nconf,
+        "train_shapes": int(len(Xtr)), "heldout_shapes": int(len(Xte)),
+        "timing_seconds": t_collect, "train_seconds": t_train,
+        "training": {"steps": args.steps, "lr": args.lr, "loss_weight_sharpness": args.sharpness,
+                     "fit_shapes": int(len(fit)), "validation_shapes": int(n_val), "candidates": val_scores},
+        "mlp_bf16_vs_exhaustive": _ratio_stats(t_mlp, best_te),
+        "builtin_rule_vs_exhaustive": _ratio_stats(t_rule, best_te),
+        "mlp_gain_over_rule_geomean": float(np.exp(np.mean(np.log(t_rule / t_mlp)))),
+        "mlp_gain_over_rule_total": float(t_rule.sum() / t_mlp.sum()),
+        "host_dispatch_agrees_with_gpu_inference": float(np.mean(host == chosen)),
+        "max_flop": args.max_flop,
+    }
+    print(json.dumps(rep, indent=1), flush=True)
+    if args.out:
+        sel.save(args.out)
+    if args.fixture:
+        Path(args.fixture).parent.mkdir(parents=True, exist_ok=True)
+        np.savez_compressed(args.fixture, X=Xte.astype(np.float32), T=Tte.astype(np.float32),
+                            shapes=np.array([_enc_plain(s) for s in te_shapes], dtype=np.int64))
     if args.report:
         Path(args.report).parent.mkdir(parents=True, exist_ok=True)
         Path(args.report).write_text(json.dumps(rep, indent=1))

@@ -20,6 +20,7 @@ from . import _lib
 
 # The learned tile-configuration selector trained on MI355X timings.
 DEFAULT_SELECTOR = Path(__file__).resolve().parent / "selector_mi355x.npz"
+DEFAULT_PLAIN_SELECTOR = Path(__file__).resolve().parent / "selector_plain_mi355x.npz"
 
 # ---------------------------------------------------------------------------
 # Enums (include/operations/blas3_trees.h:59, extension/reduction.h:40)
@@ -111,18 +112,26 @@ class SB_Handle:
     ``selector``: "default" installs the learned tile-configuration selector
     trained on MI355X (``selector_mi355x.npz`` next to this module) for the
     BF16 GEMM engine; None keeps the built-in rule; a path installs that
-    network (``portblas_amd.tuning.Selector.save`` format)."""
+    network (``portblas_amd.tuning.Selector.save`` format).
 
-    def __init__(self, stream: Any = None, selector: Any = "default"):
+    ``plain_selector``: the same for the FP32 / FP64 engine behind _gemm
+    (``selector_plain_mi355x.npz``)."""
+
+    def __init__(self, stream: Any = None, selector: Any = "default", plain_selector: Any = "default"):
         self._L = _lib.load()
         self._h = ctypes.c_void_p()
         check(self._L.pb_handle_create(self._stream_ptr(stream), ctypes.byref(self._h)))
-        if selector is not None:
+        if selector is not None or plain_selector is not None:
             from . import tuning
 
-            path = DEFAULT_SELECTOR if selector == "default" else Path(selector)
-            if selector != "default" or path.exists():
-                tuning.Selector.load(path).install(self)
+            if selector is not None:
+                path = DEFAULT_SELECTOR if selector == "default" else Path(selector)
+                if selector != "default" or path.exists():
+                    tuning.Selector.load(path).install(self)
+            if plain_selector is not None:
+                path = DEFAULT_PLAIN_SELECTOR if plain_selector == "default" else Path(plain_selector)
+                if plain_selector != "default" or path.exists():
+                    tuning.install_plain(tuning.Selector.load(path), self)
 
     @staticmethod
     def _stream_ptr(stream: Any) -> int:

@@ -169,7 +169,8 @@ pb_status_t launch_simt(pb_handle_t h, GemmProblem<float> p, bool kca, bool kcb,
 
 // Tile shape and split count for the plain MFMA path: big tiles when there
 // are enough of them to fill the chip, else 64 x 64 tiles, else split K.
-struct Plan { bool big; int splits; };
+// small: the small-problem batch kernel (one problem per workgroup).
+struct Plan { bool big; int splits; bool small = false; };
 
 inline Plan plan_plain(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t batch, int bk) {
   const int64_t cus = h->num_cus;
@@ -188,26 +189,97 @@ inline Plan plan_plain(pb_handle_t h, int64_t M, int64_t N, int64_t K, int64_t b
   return {false, int(s)};
 }
 
+}  // namespace pbgemm
+
+// ---------------------------------------------------------------------------
+// The plain engine's configurations, the autotuner's search space for FP32
+// and FP64 _gemm: 2 tile shapes (128 x 128, 64 x 64) x 8 split-K factors,
+// then the small-problem batch kernel (only for M, N <= 64 with at least one
+// problem per CU).  The reference's counterpart is the hand-written choice
+// of src/interface/blas3/backend/amd_gpu.hpp:52-288, tuned offline by
+// tools/auto_tuner/include/gemm_tuner.hpp:99-180.
+// ---------------------------------------------------------------------------
+namespace pbgemm {
+
+constexpr int kPlainSplits[] = {1, 2, 4, 8, 16, 32, 64, 128};
+constexpr int kPlainNumSplits = 8;
+constexpr int kPlainSmall = 2 * kPlainNumSplits;
+constexpr int kPlainConfigs = kPlainSmall + 1;
+
+int selector_eval(const pb_selector_state& s, const float f[8]);
+
+// Features of an FP32 (dtype 1) or FP64 (dtype 2) plain GEMM, in the
+// selector's order: log2 M, N, K, transa, transb, element type, log2 batch,
+// log2 of A's leading-dimension padding.
+inline void plain_features(int dtype, char ta, char tb, int64_t M, int64_t N, int64_t K, int64_t batch,
+                           int64_t lda, float f[8]) {
+  auto lg = [](double v) { return float(std::log2(v > 1.0 ? v : 1.0)); };
+  f[0] = lg(double(M));
+  f[1] = lg(double(N));
+  f[2] = lg(double(K));
+  f[3] = ta == 't' ? 1.f : 0.f;
+  f[4] = tb == 't' ? 1.f : 0.f;
+  f[5] = float(dtype);
+  f[6] = lg(double(batch));
+  const int64_t arows = ta == 'n' ? M : K;
+  f[7] = lg(double(lda) / double(arows > 0 ? arows : 1));
+}
+
+// Tiles x splits of plain config `id` (< kPlainSmall) for an M x N x batch problem.
+inline int64_t plain_split_tiles(pb_handle_t, int64_t M, int64_t N, int64_t batch, int id) {
+  const int64_t bm = id < kPlainNumSplits ? 128 : 64;
+  const int64_t tiles = pb::ceil_div(M, bm) * pb::ceil_div(N, bm) * batch;
+  const int splits = kPlainSplits[id % kPlainNumSplits];
+  return splits > 1 ? tiles * splits : 0;
+}
+
+inline Plan plain_config_plan(int id) {
+  if (id == kPlainSmall) return {false, 1, true};
+  return {id < kPlainNumSplits, kPlainSplits[id % kPlainNumSplits], false};
+}
+
+// The plan run_plain uses: `forced` (a configuration id) when given and
+// valid for the shape, else the handle's plain selector when one is
+// installed (forced == kRuleOnly skips it), else the built-in rule.
+constexpr int kRuleOnly = -2;
+
+template <typename T>
+Plan choose_plain(pb_handle_t h, const GemmCall<T>& g, int forced = -1) {
+  const bool small_ok = small::eligible(h, g.M, g.N, g.batch, g.interleaved);
+  int id = forced;
+  if (id == -1 && h->plain_selector.n_layers > 0) {
+    float f[8];
+    plain_features(sizeof(T) == 4 ? 1 : 2, g.ta, g.tb, g.M, g.N, g.K, g.batch, g.lda, f);
+    id = selector_eval(h->plain_selector, f);
+    // The tuner never times splits beyond 16 split tiles per CU (their
+    // partials only cost memory traffic); such a pick gives way to the rule.
+    if (id >= 0 && id < kPlainSmall && plain_split_tiles(h, g.M, g.N, g.batch, id) > 16 * int64_t(h->num_cus))
+      id = -1;
+  }
+  if (id >= 0 && id < kPlainConfigs && (id != kPlainSmall || small_ok)) return plain_config_plan(id);
+  if (small_ok) return {false, 1, true};
+  return plan_plain(h, g.M, g.N, g.K, g.batch, sizeof(T) == 4 ? 32 : 16);
+}
+
 // Split-K partial bytes run_plain(h, g, false) needs, so that a caller can put
 // its own data after them in one workspace allocation (0 without a split).
 template <typename T>
-size_t plain_partial_bytes(pb_handle_t h, const GemmCall<T>& g) {
-  if (small::eligible(h, g.M, g.N, g.batch, g.interleaved)) return 0;
+size_t plain_partial_bytes(pb_handle_t h, const GemmCall<T>& g, int forced = -1) {
+  const Plan pl = choose_plain<T>(h, g, forced);
+  if (pl.small) return 0;
   if constexpr (sizeof(T) == 4) {
-    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 32);
     return pl.big ? splitk_bytes<T, F128>(g.M, g.N, g.K, g.batch, pl.splits)
                   : splitk_bytes<T, F64>(g.M, g.N, g.K, g.batch, pl.splits);
   } else {
-    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 16);
     return pl.big ? splitk_bytes<T, D128>(g.M, g.N, g.K, g.batch, pl.splits)
                   : splitk_bytes<T, D64>(g.M, g.N, g.K, g.batch, pl.splits);
   }
 }
 
 // `partial`: caller-provided split-K partial buffer (plain_partial_bytes), or
-// null for the handle workspace.
+// null for the handle workspace.  `forced`: a plain configuration id, or -1.
 template <typename T>
-pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt, T* partial = nullptr) {
+pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt, T* partial = nullptr, int forced = -1) {
   GemmProblem<T> p{};
   p.M = g.M; p.N = g.N; p.K = g.K;
   p.alpha = g.alpha; p.beta = g.beta;
@@ -224,16 +296,17 @@ pb_status_t run_plain(pb_handle_t h, const GemmCall<T>& g, bool simt, T* partial
   p.c_vec = !g.interleaved && aligned16(g.c) && g.ldc % 4 == 0 &&
             (g.batch == 1 || g.sc % 4 == 0);
   const bool kca = g.ta == 't', kcb = g.tb == 'n';
-  if (!simt && small::eligible(h, g.M, g.N, g.batch, g.interleaved))
-    return small::launch<T>(h, p, vec);
   if constexpr (sizeof(T) == 4) {
     if (simt) return launch_simt(h, p, kca, kcb, vec);
-    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 32);
+  } else {
+    (void)simt;
+  }
+  const Plan pl = choose_plain<T>(h, g, forced);
+  if (pl.small) return small::launch<T>(h, p, vec);
+  if constexpr (sizeof(T) == 4) {
     if (pl.big) return launch_gemm<T, F128>(h, p, kca, kcb, vec, pl.splits);
     return launch_gemm<T, F64>(h, p, kca, kcb, vec, pl.splits);
   } else {
-    (void)simt;
-    const Plan pl = plan_plain(h, g.M, g.N, g.K, g.batch, 16);
     if (pl.big) return launch_gemm<T, D128>(h, p, kca, kcb, vec, pl.splits);
     return launch_gemm<T, D64>(h, p, kca, kcb, vec, pl.splits);
   }
@@ -713,6 +786,79 @@ using namespace pbgemm;
 
 extern "C" {
 
+int pb_tune_num_plain_configs(void) { return kPlainConfigs; }
+
+const char* pb_tune_plain_config_name(int id) {
+  static const char* names[kPlainConfigs] = {
+      "128x128/splitk1", "128x128/splitk2", "128x128/splitk4", "128x128/splitk8", "128x128/splitk16",
+      "128x128/splitk32", "128x128/splitk64", "128x128/splitk128", "64x64/splitk1", "64x64/splitk2",
+      "64x64/splitk4", "64x64/splitk8", "64x64/splitk16", "64x64/splitk32", "64x64/splitk64",
+      "64x64/splitk128", "small-batch"};
+  return (id >= 0 && id < kPlainConfigs) ? names[id] : "";
+}
+
+pb_status_t pb_gemm_plain_config(pb_handle_t h, int dtype, int id, char ta, char tb, int64_t m, int64_t n,
+                                 int64_t k, double alpha, const void* a, int64_t lda, const void* b,
+                                 int64_t ldb, double beta, void* c, int64_t ldc, int64_t batch) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  PB_REQUIRE(dtype == 1 || dtype == 2, "dtype must be 1 (FP32) or 2 (FP64)");
+  PB_REQUIRE(id >= -1 && id < kPlainConfigs, "config id %d out of range", id);
+  auto run = [&](auto zero) -> pb_status_t {
+    using T = decltype(zero);
+    int64_t sa = 0, sb = 0, sc = 0;
+    default_strides(ta, tb, m, n, k, lda, ldb, ldc, PB_BATCH_STRIDED, sa, sb, sc);
+    GemmCall<T> g;
+    bool done = false;
+    PB_TRY(gemm_front<T>(h, ta, tb, m, n, k, T(alpha), static_cast<const T*>(a), lda, sa,
+                         static_cast<const T*>(b), ldb, sb, T(beta), static_cast<T*>(c), ldc, sc, batch,
+                         PB_BATCH_STRIDED, g, done));
+    if (done) return PB_STATUS_SUCCESS;
+    if (g.alpha == T(0)) return alpha_zero<T>(h, g.M, g.N, g.beta, g.c, g.ldc, g.sc, g.batch, false);
+    if (id == kPlainSmall && !small::eligible(h, g.M, g.N, g.batch, g.interleaved))
+      return pb::set_error(PB_STATUS_NOT_SUPPORTED, "the small-batch kernel needs M, N <= 64 and a batch "
+                           "of at least one problem per CU");
+    return run_plain<T>(h, g, false, nullptr, id < 0 ? kRuleOnly : id);
+  };
+  return dtype == 1 ? run(0.f) : run(0.0);
+}
+
+pb_status_t pb_tune_time_plain(pb_handle_t h, int dtype, int id, char ta, char tb, int64_t m, int64_t n,
+                               int64_t k, const void* a, int64_t lda, const void* b, int64_t ldb, void* c,
+                               int64_t ldc, int64_t batch, int warmup, int reps, float* usec_out) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  PB_REQUIRE(usec_out != nullptr && reps > 0 && warmup >= 0, "bad timing arguments");
+  for (int i = 0; i < warmup; ++i)
+    PB_TRY(pb_gemm_plain_config(h, dtype, id, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, c, ldc, batch));
+  hipEvent_t e0, e1;
+  PB_CHECK_HIP(hipEventCreate(&e0));
+  PB_CHECK_HIP(hipEventCreate(&e1));
+  PB_CHECK_HIP(hipEventRecord(e0, h->stream));
+  pb_status_t st = PB_STATUS_SUCCESS;
+  for (int i = 0; i < reps && st == PB_STATUS_SUCCESS; ++i)
+    st = pb_gemm_plain_config(h, dtype, id, ta, tb, m, n, k, 1.0, a, lda, b, ldb, 0.0, c, ldc, batch);
+  hipError_t e = hipEventRecord(e1, h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(e1);
+  float ms = 0.f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (st != PB_STATUS_SUCCESS) return st;
+  if (e != hipSuccess)
+    return pb::set_error(PB_STATUS_DEVICE_ERROR, "timing failed: %s", hipGetErrorString(e));
+  *usec_out = 1000.f * ms / float(reps);
+  return PB_STATUS_SUCCESS;
+}
+
+int pb_tune_select_plain(pb_handle_t h, int dtype, char ta, char tb, int64_t m, int64_t n, int64_t k,
+                         int64_t batch, int64_t lda) {
+  if (!h || h->plain_selector.n_layers == 0) return -1;
+  float f[8];
+  plain_features(dtype, pb::norm_trans(ta), pb::norm_trans(tb), m, n, k, batch, lda, f);
+  return selector_eval(h->plain_selector, f);
+}
+
 pb_status_t pb_sgemm_noalloc(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
                              float alpha, const float* a, int64_t lda, const float* b,
                              int64_t ldb, float beta, float* c, int64_t ldc) {
@@ -725,8 +871,15 @@ pb_status_t pb_sgemm_noalloc(pb_handle_t h, char ta, char tb, int64_t m, int64_t
   if (done) return PB_STATUS_SUCCESS;
   if (g.alpha == 0.f)
     return alpha_zero<float>(h, g.M, g.N, g.beta, g.c, g.ldc, g.sc, g.batch, g.interleaved);
-  // Same plan and kernels as pb_sgemm; only the partials' home differs.
-  const size_t need = plain_partial_bytes<float>(h, g);
+  // Same plan and kernels as pb_sgemm; only the partials' home differs.  A
+  // plan from an installed selector that needs more partials than the
+  // scratch holds gives way to the built-in rule (at most 2 CUs - 1 tiles).
+  int forced = -1;
+  size_t need = plain_partial_bytes<float>(h, g);
+  if (need > kNoAllocScratchBytes) {
+    forced = kRuleOnly;
+    need = plain_partial_bytes<float>(h, g, forced);
+  }
   if (need > kNoAllocScratchBytes)
     return pb::set_error(PB_STATUS_ALLOC_FAILED,
                          "pb_sgemm_noalloc: %zu bytes of split-K partials exceed the %zu-byte "
@@ -734,7 +887,7 @@ pb_status_t pb_sgemm_noalloc(pb_handle_t h, char ta, char tb, int64_t m, int64_t
                          need, kNoAllocScratchBytes);
   void* part = nullptr;
   PB_TRY(scratch_address(h->device, &part));
-  return run_plain<float>(h, g, false, static_cast<float*>(part));
+  return run_plain<float>(h, g, false, static_cast<float*>(part), forced);
 }
 
 pb_status_t pb_cgemm(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k, const float* alpha,

@@ -272,7 +272,15 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
 // chunks, so one barrier per chunk separates a buffer's reads from its next
 // writes.
 // ---------------------------------------------------------------------------
-constexpr int kRD = 4;  // K chunks in flight per workgroup
+// Ring geometry: K chunk kRK (64: every row piece of a K-contiguous operand,
+// and every column piece of an M/N-contiguous one, is a whole 128-byte line;
+// with 32 each 128-byte line was fetched twice, once per half), kRD chunks in
+// flight per workgroup.
+constexpr int kRK = 64;
+constexpr int kRD = 4;
+constexpr int kRP = kRK + 8;              // LDS row pitch ([row][k], elements)
+constexpr int kRG = kRK / 8;              // 16-byte groups per K-contiguous row of a chunk
+constexpr int kRIt = kLT * kRG / 256;     // 16-byte loads per thread and operand
 
 __device__ __attribute__((aligned(16))) const uint32_t kZero16[4] = {0u, 0u, 0u, 0u};
 
@@ -295,28 +303,54 @@ __device__ __forceinline__ void ring_set_tile(RingCursor& c, const DevProblem* p
   c.n0 = int64_t(t.z) * kLT;
   c.i = i;
   c.kc = 0;
-  c.nk = P.K > 0 ? int((P.K + kLK - 1) / kLK) : 1;
+  c.nk = P.K > 0 ? int((P.K + kRK - 1) / kRK) : 1;
   c.prob = t.x;
 }
 
-// One 16-byte load per operand for chunk c.kc of tile c.i (zero block for
-// the groups outside the operand).
+// Group e of a chunk: KC -> row e / kRG, k group (e % kRG) * 8; else k
+// e / 8, row group (e % 8) * 8.  One 16-byte load (a zero block outside the
+// operand).
+template <bool KC>
+__device__ __forceinline__ u32x4 ring_group(const uint16_t* x, int64_t rs, int64_t ks, int64_t rows, int64_t K,
+                                            int64_t r0, int64_t k0, int e) {
+  const int64_t r = KC ? r0 + e / kRG : r0 + (e % 8) * 8;
+  const int64_t k = KC ? k0 + (e % kRG) * 8 : k0 + e / 8;
+  const uint16_t* p = (r < rows && k < K) ? x + r * rs + k * ks : reinterpret_cast<const uint16_t*>(kZero16);
+  return *reinterpret_cast<const u32x4*>(p);
+}
+
 template <bool KCA, bool KCB>
-__device__ __forceinline__ void ring_load(const RingCursor& c, u32x4& va, u32x4& vb) {
-  const int e = threadIdx.x;
-  const int64_t k0 = int64_t(c.kc) * kLK;
-  const uint16_t* zero = reinterpret_cast<const uint16_t*>(kZero16);
-  {
-    const int64_t r = KCA ? c.m0 + e / kLG : c.m0 + (e % 8) * 8;
-    const int64_t k = KCA ? k0 + (e % kLG) * 8 : k0 + e / 8;
-    const uint16_t* p = (r < c.M && k < c.K) ? c.a + r * c.a_rs + k * c.a_ks : zero;
-    va = *reinterpret_cast<const u32x4*>(p);
+__device__ __forceinline__ void ring_load(const RingCursor& c, u32x4 (&va)[kRIt], u32x4 (&vb)[kRIt]) {
+  const int64_t k0 = int64_t(c.kc) * kRK;
+#pragma unroll
+  for (int it = 0; it < kRIt; ++it) {
+    const int e = threadIdx.x + 256 * it;
+    va[it] = ring_group<KCA>(c.a, c.a_rs, c.a_ks, c.M, c.K, c.m0, k0, e);
+    vb[it] = ring_group<KCB>(c.b, c.b_rs, c.b_ks, c.N, c.K, c.n0, k0, e);
   }
-  {
-    const int64_t r = KCB ? c.n0 + e / kLG : c.n0 + (e % 8) * 8;
-    const int64_t k = KCB ? k0 + (e % kLG) * 8 : k0 + e / 8;
-    const uint16_t* p = (r < c.N && k < c.K) ? c.b + r * c.b_rs + k * c.b_ks : zero;
-    vb = *reinterpret_cast<const u32x4*>(p);
+}
+
+// 16-byte chunk c of LDS row r sits at chunk c ^ ring_swz(r).  With the
+// 144-byte pitch, 16 consecutive rows at one chunk fall on 16 different
+// 16-byte slots of a 256-byte window (one swizzle value per 16-row block keeps
+// the fragment reads conflict-free), and the swizzle spreads the transposing
+// 2-byte stores (8 rows 8 apart, which the pitch alone puts on 2 slots) over 8.
+__device__ __forceinline__ int ring_swz(int r) { return (r >> 4) & (kRG - 1); }
+
+template <bool KC>
+__device__ __forceinline__ void ring_store(uint16_t (*dst)[kRP], int e, const u32x4& v) {
+  if constexpr (KC) {
+    const int r = e / kRG;
+    *reinterpret_cast<u32x4*>(&dst[r][((e % kRG) ^ ring_swz(r)) * 8]) = v;
+  } else {
+    uint16_t h[8];
+    __builtin_memcpy(h, &v, 16);
+    const int k = e / 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int r = (e % 8) * 8 + i;
+      dst[r][((k / 8) ^ ring_swz(r)) * 8 + k % 8] = h[i];
+    }
   }
 }
 
@@ -324,8 +358,8 @@ template <bool KCA, bool KCB>
 __global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __restrict__ probs,
                                                            const int4* __restrict__ tiles,
                                                            const int* __restrict__ starts) {
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][kLT][kLP];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kLT][kLP];
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][kLT][kRP];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kLT][kRP];
   const int begin = starts[blockIdx.x], end = starts[blockIdx.x + 1];
   if (begin >= end) return;
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
@@ -339,7 +373,7 @@ __global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __r
     else if (pc.i + 1 < end) ring_set_tile(pc, probs, tiles, pc.i + 1);
     // else: stay on the last chunk (re-read, never consumed)
   };
-  u32x4 va[kRD], vb[kRD];
+  u32x4 va[kRD][kRIt], vb[kRD][kRIt];
 #pragma unroll
   for (int s = 0; s < kRD; ++s) {
     ring_load<KCA, KCB>(pc, va[s], vb[s]);
@@ -356,20 +390,20 @@ __global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __r
     constexpr int S = decltype(slot)::value;
     constexpr int BUF = S & 1;
 #pragma unroll
-    for (int it = 0; it < kLIt; ++it) {
+    for (int it = 0; it < kRIt; ++it) {
       const int e = threadIdx.x + 256 * it;
-      lean_store<KCA>(As[BUF], e, va[S]);
-      lean_store<KCB>(Bs[BUF], e, vb[S]);
+      ring_store<KCA>(As[BUF], e, va[S][it]);
+      ring_store<KCB>(Bs[BUF], e, vb[S][it]);
     }
     __syncthreads();
     ring_load<KCA, KCB>(pc, va[S], vb[S]);
     advance_producer();
 #pragma unroll
-    for (int kk = 0; kk < kLK; kk += 16) {
+    for (int kk = 0; kk < kRK; kk += 16) {
       bf16x8 a, b;
       const int ra = wm + lane % 32, rb = wn + lane % 32, c = kk / 8 + lane / 32;
-      __builtin_memcpy(&a, &As[BUF][ra][(c ^ lean_swz(ra)) * 8], 16);
-      __builtin_memcpy(&b, &Bs[BUF][rb][(c ^ lean_swz(rb)) * 8], 16);
+      __builtin_memcpy(&a, &As[BUF][ra][(c ^ ring_swz(ra)) * 8], 16);
+      __builtin_memcpy(&b, &Bs[BUF][rb][(c ^ ring_swz(rb)) * 8], 16);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
     if (cc.kc + 1 < cc.nk) {
@@ -485,13 +519,13 @@ void launch_lean(pb_handle_t h, const Launch& L) {
 // lists, concatenated, and their offsets.
 constexpr int kXcds = 8;
 
-void balance_tiles(const std::vector<DevProblem>& v, int grid, std::vector<int4>& list,
+void balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, std::vector<int4>& list,
                    std::vector<int>& starts) {
   struct Item { int64_t cost; int prob; };
   std::vector<Item> probs_by_cost;
   int64_t total = 0;
   for (size_t i = 0; i < v.size(); ++i) {
-    probs_by_cost.push_back({pb::ceil_div(std::max<int64_t>(v[i].K, 1), kLK) + 1, int(i)});
+    probs_by_cost.push_back({pb::ceil_div(std::max<int64_t>(v[i].K, 1), kchunk) + 1, int(i)});
     total += probs_by_cost.back().cost * int64_t(v[i].tiles_m) * v[i].tiles_n;
   }
   std::stable_sort(probs_by_cost.begin(), probs_by_cost.end(), [&](const Item& x, const Item& y) {
@@ -684,7 +718,7 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
       L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, lean_per_cu(lay) * h->num_cus)));
       std::vector<int4> list;
       std::vector<int> starts;
-      balance_tiles(v, L.grid, list, starts);
+      balance_tiles(v, L.grid, L.ring ? kRK : kLK, list, starts);
       void* dl = nullptr;
       void* ds = nullptr;
       PB_TRY(upload(list.data(), list.size() * sizeof(int4), &dl));

@@ -196,12 +196,10 @@ void selector_features(char ta, char tb, int64_t M, int64_t N, int64_t K, int64_
   f[7] = 0.f;  // leading-dimension padding (log2 of lda / rows); packed = 0
 }
 
-int selector_choose(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64_t K,
-                    int64_t batch) {
-  const pb_selector_state& s = h->selector;
+// The configuration an installed network picks for features f (its
+// argmin of predicted relative runtime), or -1 when none is installed.
+int selector_eval(const pb_selector_state& s, const float f[8]) {
   if (s.n_layers == 0) return -1;
-  float f[8];
-  selector_features(ta, tb, M, N, K, batch, f);
   std::vector<float> x(size_t(s.sizes[0]), 0.f), y;
   for (int i = 0; i < s.sizes[0] && i < 8; ++i) x[i] = (f[i] - s.mean[i]) / s.stdv[i];
   for (int l = 0; l < s.n_layers; ++l) {
@@ -222,6 +220,18 @@ int selector_choose(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64
   return best;
 }
 
+int selector_choose(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, int64_t K,
+                    int64_t batch) {
+  float f[8];
+  selector_features(ta, tb, M, N, K, batch, f);
+  return selector_eval(h->selector, f);
+}
+
+// Validates and converts a network for a handle (weights rounded to BF16).
+pb_status_t selector_state(int n_layers, const int* sizes, const float* const* weights,
+                           const float* const* biases, const float* feat_mean, const float* feat_std,
+                           int outputs, pb_selector_state& s);
+
 }  // namespace pbgemm
 
 extern "C" {
@@ -232,12 +242,36 @@ pb_status_t pb_tune_set_selector(pb_handle_t h, int n_layers, const int* sizes,
   pb::clear_error();
   PB_TRY(pb::enter(h));
   pb_selector_state s;
-  if (n_layers == 0) {
-    h->selector = s;
-    return PB_STATUS_SUCCESS;
-  }
+  PB_TRY(pbgemm::selector_state(n_layers, sizes, weights, biases, feat_mean, feat_std, 0, s));
+  h->selector = std::move(s);
+  return PB_STATUS_SUCCESS;
+}
+
+pb_status_t pb_tune_set_plain_selector(pb_handle_t h, int n_layers, const int* sizes,
+                                       const float* const* weights, const float* const* biases,
+                                       const float* feat_mean, const float* feat_std) {
+  pb::clear_error();
+  PB_TRY(pb::enter(h));
+  pb_selector_state s;
+  PB_TRY(pbgemm::selector_state(n_layers, sizes, weights, biases, feat_mean, feat_std,
+                                pb_tune_num_plain_configs(), s));
+  h->plain_selector = std::move(s);
+  return PB_STATUS_SUCCESS;
+}
+
+}  // extern "C"
+
+namespace pbgemm {
+
+pb_status_t selector_state(int n_layers, const int* sizes, const float* const* weights,
+                           const float* const* biases, const float* feat_mean, const float* feat_std,
+                           int outputs, pb_selector_state& s) {
+  s = pb_selector_state();
+  if (n_layers == 0) return PB_STATUS_SUCCESS;
   PB_TRY(check_sizes(n_layers, sizes));
   PB_REQUIRE(sizes[0] == 8, "the selector takes 8 features (got %d)", sizes[0]);
+  PB_REQUIRE(outputs == 0 || sizes[n_layers] == outputs, "the selector must have %d outputs (got %d)", outputs,
+             sizes[n_layers]);
   PB_REQUIRE(weights && biases && feat_mean && feat_std, "null selector array");
   s.n_layers = n_layers;
   s.sizes.assign(sizes, sizes + n_layers + 1);
@@ -252,9 +286,12 @@ pb_status_t pb_tune_set_selector(pb_handle_t h, int n_layers, const int* sizes,
   s.stdv.assign(feat_std, feat_std + 8);
   for (float& v : s.stdv)
     if (!(v > 0.f)) v = 1.f;
-  h->selector = std::move(s);
   return PB_STATUS_SUCCESS;
 }
+
+}  // namespace pbgemm
+
+extern "C" {
 
 int pb_tune_select(pb_handle_t h, char ta, char tb, int64_t m, int64_t n, int64_t k,
                    int64_t batch) {

@@ -98,7 +98,8 @@ struct pb_handle_s {
   // Calls that overlap on one handle are still unsupported: they share the
   // stream, the workspace contents and the counter arena.
   std::mutex mu;
-  pb_selector_state selector;
+  pb_selector_state selector;        // BF16-input engine (40 configurations)
+  pb_selector_state plain_selector;  // FP32 / FP64 plain engine (kPlainConfigs)
 };
 
 namespace pb {

@@ -255,6 +255,145 @@ def select(handle, ta, tb, M, N, K, batch) -> int:
     return int(_lib.load().pb_tune_select(handle.handle, ta.encode(), tb.encode(), M, N, K, batch))
 
 
+# ---------------------------------------------------------------------------
+# The plain FP32 / FP64 engine (pb_sgemm / pb_dgemm and their batched forms)
+# ---------------------------------------------------------------------------
+PLAIN_DTYPES = {1: "float32", 2: "float64"}
+PLAIN_SPLITS = (1, 2, 4, 8, 16, 32, 64, 128)
+
+
+def num_plain_configs() -> int:
+    return int(_lib.load().pb_tune_num_plain_configs())
+
+
+def plain_config_name(i: int) -> str:
+    return _lib.load().pb_tune_plain_config_name(i).decode()
+
+
+def plain_features(dtype: int, ta: str, tb: str, M: int, N: int, K: int, batch: int, lda: int) -> np.ndarray:
+    """The library's feature order (gemm.hip: plain_features)."""
+    lg = lambda v: math.log2(max(float(v), 1.0))
+    arows = M if ta == "n" else K
+    return np.array([lg(M), lg(N), lg(K), float(ta == "t"), float(tb == "t"), float(dtype), lg(batch),
+                     lg(lda / max(arows, 1))], dtype=np.float32)
+
+
+def plain_split_tiles(M: int, N: int, batch: int, cid: int) -> int:
+    """Tiles x splits of plain config `cid`, as the library counts them."""
+    bm = 128 if cid < len(PLAIN_SPLITS) else 64
+    tiles = -(-M // bm) * -(-N // bm) * batch
+    splits = PLAIN_SPLITS[cid % len(PLAIN_SPLITS)]
+    return tiles * splits if splits > 1 else 0
+
+
+def sample_plain_shapes(n: int, seed: int, lo: int = 16, hi: int = 16384, max_batch: int = 256,
+                        max_flop: float = 2.0 ** 35, max_elems: int = 2 ** 26):
+    """(dtype, ta, tb, M, N, K, batch, lda): dtype FP32 or FP64 at random, M,
+    N, K log-uniform in [lo, hi], batch uniform in {1..max_batch}, random
+    transposes, lda padded beyond the rows of A in a third of the shapes;
+    redrawn while work or footprint exceeds the caps."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        M, N, K = (int(round(math.exp(rng.uniform(math.log(lo), math.log(hi))))) for _ in range(3))
+        batch = int(rng.integers(1, max_batch + 1))
+        if 2.0 * M * N * K * batch > max_flop:
+            continue
+        ta = "t" if rng.random() < 0.5 else "n"
+        tb = "t" if rng.random() < 0.5 else "n"
+        arows, acols = (M, K) if ta == "n" else (K, M)
+        lda = arows + (int(rng.integers(1, 65)) if rng.random() < 1 / 3 else 0)
+        if max(lda * acols, K * N, M * N) * batch > max_elems:
+            continue
+        out.append((int(rng.integers(1, 3)), ta, tb, M, N, K, batch, lda))
+    return out
+
+
+class PlainAutotuner:
+    """Exhaustive timing of every plain configuration for FP32 and FP64
+    shapes, plus the built-in rule (the last column).  Configurations whose
+    split tiles exceed 16 per CU, and the small-batch kernel where it cannot
+    run, are not timed (inf)."""
+
+    def __init__(self, handle, max_elems: int = 2 ** 26):
+        import torch
+
+        self.h = handle
+        self.L = _lib.load()
+        self.cus = handle.get_num_compute_units()
+        self.buf = {}
+        for dt, name in PLAIN_DTYPES.items():
+            tdt = getattr(torch, name)
+            self.buf[dt] = (torch.empty(max_elems, dtype=tdt, device="cuda").uniform_(-1, 1),
+                            torch.empty(max_elems, dtype=tdt, device="cuda").uniform_(-1, 1),
+                            torch.zeros(max_elems, dtype=tdt, device="cuda"))
+        self.max_elems = max_elems
+
+    def time_all(self, shape, warmup: int = 1, reps: int = 3) -> np.ndarray:
+        dt, ta, tb, M, N, K, batch, lda = shape
+        A, B, C = self.buf[dt]
+        ldb = K if tb == "n" else N
+        nconf = num_plain_configs()
+        out = np.full(nconf + 1, np.inf)
+        us = ctypes.c_float()
+        small_ok = M <= 64 and N <= 64 and batch >= self.cus
+        for cid in list(range(nconf)) + [-1]:
+            if 0 <= cid < nconf - 1 and plain_split_tiles(M, N, batch, cid) > 16 * self.cus:
+                continue
+            if cid == nconf - 1 and not small_ok:
+                continue
+            check(self.L.pb_tune_time_plain(self.h.handle, dt, cid, ta.encode(), tb.encode(), M, N, K,
+                                            A.data_ptr(), lda, B.data_ptr(), ldb, C.data_ptr(), M, batch,
+                                            warmup, reps, ctypes.byref(us)))
+            out[cid if cid >= 0 else nconf] = us.value
+        return out
+
+
+def collect_plain(tuner: PlainAutotuner, shapes, progress=None, reps: int = 3):
+    """Features (n x 8) and timings (n x configs + 1: the rule last, us)."""
+    X = np.stack([plain_features(*s) for s in shapes])
+    T = np.stack([tuner.time_all(s, reps=reps) for s in _progress(shapes, progress)])
+    return X, T
+
+
+def _progress(items, every):
+    for i, it in enumerate(items):
+        yield it
+        if every and (i + 1) % every == 0:
+            print(f"[tune] {i + 1}/{len(items)} shapes timed", flush=True)
+
+
+def install_plain(selector: "Selector", handle) -> None:
+    """Make the handle pick FP32 / FP64 plain configs with this network."""
+    L = _lib.load()
+    sizes_c = (ctypes.c_int * len(selector.sizes))(*selector.sizes)
+    ws = [np.ascontiguousarray(w, np.float32) for w in selector.ws]
+    bs = [np.ascontiguousarray(b, np.float32) for b in selector.bs]
+    wp = (ctypes.c_void_p * len(ws))(*[w.ctypes.data for w in ws])
+    bp = (ctypes.c_void_p * len(bs))(*[b.ctypes.data for b in bs])
+    mean = np.ascontiguousarray(selector.mean, np.float32)
+    std = np.ascontiguousarray(selector.std, np.float32)
+    check(L.pb_tune_set_plain_selector(handle.handle, len(ws), sizes_c, wp, bp, mean.ctypes.data,
+                                       std.ctypes.data))
+
+
+def uninstall_plain(handle) -> None:
+    check(_lib.load().pb_tune_set_plain_selector(handle.handle, 0, None, None, None, None, None))
+
+
+def select_plain(handle, dt, ta, tb, M, N, K, batch, lda) -> int:
+    return int(_lib.load().pb_tune_select_plain(handle.handle, dt, ta.encode(), tb.encode(), M, N, K, batch, lda))
+
+
+def plain_time_of_choice(chosen: np.ndarray, T: np.ndarray) -> np.ndarray:
+    """Time each shape takes with the given configs: where a config was not
+    timed (the library would not run it and falls back to the rule), the
+    rule's time."""
+    rows = np.arange(T.shape[0])
+    t = T[rows, chosen]
+    return np.where(np.isfinite(t), t, T[:, -1])
+
+
 def evaluate(chosen: np.ndarray, T: np.ndarray) -> dict:
     """Quality of a choice of config per shape against exhaustive search."""
     rows = np.arange(T.shape[0])
