@@ -126,7 +126,9 @@ pb_status_t pb_dgemm(pb_handle_t h, char transa, char transb, int64_t m, int64_t
 /* Allocation-free pb_sgemm.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  Same arguments,
- * validation, plan, kernels and results as pb_sgemm, but the split-K partial
+ * validation, plan, kernels and results as pb_sgemm (except where an installed
+ * plain selector picks a split whose partials exceed the array: then the
+ * built-in rule's plan), but the split-K partial
  * tiles live in a 16 MiB array the library holds on each GPU from load to
  * process exit, and the call never grows the handle workspace and never
  * allocates.  A plan needing more partials than the array holds returns

@@ -115,3 +115,103 @@ def test_gpu_selector_choice_live_timing(handle, torch_cuda):
     T = np.stack([tuner.time_all(s, reps=5) for s in shapes])
     chosen = _oracle_choice(net, d["X"][pick])
     assert _evaluate(chosen, T)["geomean"] <= 1.10
+
+
+# ---------------------------------------------------------------------------
+# The FP32 / FP64 plain engine's network (portblas_amd/selector_plain_mi355x.npz)
+# and its held-out fixture (tests/golden/selector_plain_heldout_mi355x.npz,
+# written by ``python -m portblas_amd.autotune --family plain ... --fixture``:
+# 2000 FP32/FP64 shapes never trained on, M, N, K log-uniform in [16, 16384],
+# batch 1..256, random transposes and lda padding, work capped at 2^35 flop;
+# each timed in all 17 plain configurations and with the built-in rule, the
+# last column).
+# ---------------------------------------------------------------------------
+PLAIN_NET = ROOT / "portblas_amd" / "selector_plain_mi355x.npz"
+PLAIN_HELDOUT = ROOT / "tests" / "golden" / "selector_plain_heldout_mi355x.npz"
+
+
+def _load_plain():
+    z = np.load(PLAIN_NET, allow_pickle=False)
+    n = len(z["sizes"]) - 1
+    net = ([z[f"w{l}"] for l in range(n)], [z[f"b{l}"] for l in range(n)], z["mean"], z["std"],
+           [int(v) for v in z["sizes"]])
+    return net, np.load(PLAIN_HELDOUT, allow_pickle=False)
+
+
+def _plain_times(chosen, T):
+    """Time of each shape with the chosen configs; an untimed config (which
+    the library would not run, falling back to its rule) costs the rule's."""
+    t = T[np.arange(len(T)), chosen].astype(np.float64)
+    return np.where(np.isfinite(t), t, T[:, -1])
+
+
+def test_plain_selector_within_5pct_of_exhaustive_and_beats_rule():
+    """FP32/FP64 _gemm: the learned choice is within 5% of exhaustive search
+    over the 17 plain configurations (geometric mean and total time over the
+    held-out shapes) and faster than the built-in rule the library uses
+    without it."""
+    net, d = _load_plain()
+    T = d["T"].astype(np.float64)
+    nconf = T.shape[1] - 1
+    assert net[4] == [8, 64, 64, 64, nconf] and nconf == 17
+    best = T[:, :nconf].min(axis=1)
+    mlp = _plain_times(_oracle_choice(net, d["X"]), T)
+    rule = T[:, -1]
+    geo = lambda t: float(np.exp(np.mean(np.log(t / best))))
+    assert geo(mlp) <= 1.05 and mlp.sum() / best.sum() <= 1.05, (geo(mlp), mlp.sum() / best.sum())
+    assert geo(mlp) < geo(rule) - 0.03, (geo(mlp), geo(rule))
+    assert mlp.sum() < 0.95 * rule.sum(), (mlp.sum(), rule.sum())
+    # both element types and the padded lda shapes take part
+    assert set(np.unique(d["X"][:, 5])) == {1.0, 2.0} and np.mean(d["X"][:, 7] > 0) > 0.2
+
+
+def test_plain_fixture_is_consistent():
+    from portblas_amd import tuning
+
+    _, d = _load_plain()
+    T, X, shapes = d["T"], d["X"], d["shapes"]
+    assert T.shape == (2000, 18) and np.all(np.isfinite(T[:, -1])) and np.all(T[np.isfinite(T)] > 0)
+    for row, x in zip(shapes[:50], X[:50]):
+        dt, ta, tb = int(row[0]), chr(int(row[1])), chr(int(row[2]))
+        m, n, k, batch, lda = (int(v) for v in row[3:])
+        np.testing.assert_allclose(tuning.plain_features(dt, ta, tb, m, n, k, batch, lda), x, rtol=1e-6)
+
+
+@pytest.mark.gpu
+def test_gpu_plain_selector_inference_and_dispatch(handle, torch_cuda):
+    """The plain network through the BF16 inference kernel and through the
+    handle's host-side dispatch picks what the oracle picks."""
+    import portblas_amd as pb
+    from portblas_amd import tuning
+
+    net, d = _load_plain()
+    sel = tuning.Selector.load(PLAIN_NET)
+    gpu = np.argmin(sel.predict_bf16(handle, d["X"]), axis=1)
+    assert np.mean(gpu == _oracle_choice(net, d["X"])) >= 0.99
+    shapes = [(int(r[0]), chr(int(r[1])), chr(int(r[2])), *(int(v) for v in r[3:])) for r in d["shapes"][:300]]
+    h = pb.SB_Handle(plain_selector=None)
+    assert tuning.select_plain(h, *shapes[0]) == -1  # none installed: the built-in rule
+    tuning.install_plain(sel, h)
+    host = np.array([tuning.select_plain(h, *s) for s in shapes])
+    assert np.mean(host == gpu[:300]) >= 0.99
+    # SB_Handle() installs the committed network by default
+    assert np.array_equal(np.array([tuning.select_plain(handle, *s) for s in shapes[:50]]), host[:50])
+    h.close()
+
+
+@pytest.mark.gpu
+def test_gpu_plain_selector_choice_live_timing(handle, torch_cuda):
+    """Re-time 16 median-sized held-out shapes: the network's choice is within
+    10% of this run's exhaustive optimum (geometric mean)."""
+    from portblas_amd import tuning
+
+    net, d = _load_plain()
+    sizes = d["shapes"][:, 3:6].astype(np.float64).prod(axis=1) * d["shapes"][:, 6]
+    pick = np.argsort(sizes)[1000:1016]
+    shapes = [(int(r[0]), chr(int(r[1])), chr(int(r[2])), *(int(v) for v in r[3:])) for r in d["shapes"][pick]]
+    tuner = tuning.PlainAutotuner(handle)
+    T = np.stack([tuner.time_all(s, reps=5) for s in shapes])
+    chosen = _oracle_choice(net, d["X"][pick])
+    t = _plain_times(chosen, T)
+    best = T[:, :-1].min(axis=1)
+    assert float(np.exp(np.mean(np.log(t / best)))) <= 1.10

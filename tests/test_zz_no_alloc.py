@@ -1,6 +1,7 @@
 """The allocation-free SGEMM (pb_sgemm_noalloc) against pb_sgemm, through the
 C ABI.  Both run the same plan and kernels, so their results must be
-bit-identical; the split-K partials of the allocation-free call live in the
+bit-identical (with the built-in plan on both handles); the split-K partials
+of the allocation-free call live in the
 library's per-GPU scratch array instead of the handle workspace, which must
 stay unallocated.  Two split-K calls in a row, the second smaller (it reuses
 the array the first one left behind), then a call without a split, each also
@@ -38,8 +39,10 @@ def test_sgemm_noalloc_matches_sgemm(torch_cuda):
 
     torch = torch_cuda
     L = pb.load()
-    h_ref = pb.SB_Handle(torch.cuda.Stream())
-    h_na = pb.SB_Handle(torch.cuda.Stream())
+    # The built-in plan on both handles: an installed FP32 selector may pick a
+    # split whose partials exceed the array, and then only pb_sgemm uses it.
+    h_ref = pb.SB_Handle(torch.cuda.Stream(), plain_selector=None)
+    h_na = pb.SB_Handle(torch.cuda.Stream(), plain_selector=None)
     assert h_na.workspace_bytes() == 0
     for i, (ta, tb, m, n, k, alpha, beta) in enumerate(CASES):
         print(f"pb_sgemm_noalloc case {i}: {ta}{tb} {m}x{n}x{k}", flush=True)
@@ -72,7 +75,7 @@ def test_sgemm_noalloc_argument_rules(torch_cuda):
 
     torch = torch_cuda
     L = pb.load()
-    h = pb.SB_Handle(torch.cuda.Stream())
+    h = pb.SB_Handle(torch.cuda.Stream(), plain_selector=None)
     c = torch.full((16,), 3.0, device="cuda")
     # invalid transpose flag, as pb_sgemm reports it
     st = L.pb_sgemm_noalloc(h.handle, b"x", b"n", 4, 4, 4, 1.0, c.data_ptr(), 4, c.data_ptr(), 4, 0.0,
