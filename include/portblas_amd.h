@@ -482,7 +482,10 @@ pb_status_t pb_tune_time_plain(pb_handle_t h, int dtype, int id, char transa, ch
 pb_status_t pb_tune_set_plain_selector(pb_handle_t h, int n_layers, const int* layer_sizes,
                                        const float* const* weights, const float* const* biases,
                                        const float* feat_mean, const float* feat_std);
-/* The config the installed plain network picks (-1: none installed). */
+/* The config the installed plain network picks; -1 when none is installed or
+ * the shape lies outside the network's training domain (work above 2^35
+ * flop, operands above 2^26 elements or batches above 256), where the
+ * library uses its built-in rule. */
 int pb_tune_select_plain(pb_handle_t h, int dtype, char transa, char transb, int64_t m, int64_t n,
                          int64_t k, int64_t batch, int64_t lda);
 

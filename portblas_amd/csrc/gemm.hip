@@ -8,6 +8,8 @@
 #include "gemm_front.hpp"
 #include "gemm_small.hpp"
 
+#include <algorithm>
+
 namespace pbgemm {
 
 using F128 = TileShape<128, 128, 32, 64, 64>;
@@ -233,6 +235,15 @@ inline int64_t plain_split_tiles(pb_handle_t, int64_t M, int64_t N, int64_t batc
   return splits > 1 ? tiles * splits : 0;
 }
 
+// The shapes the plain network was trained and validated on: work up to
+// 2^35 flop, operands up to 2^26 elements, batches up to 256.  Outside them
+// the built-in rule decides (the network would be extrapolating).
+inline bool plain_selector_domain(int64_t M, int64_t N, int64_t K, int64_t batch) {
+  const double flop = 2.0 * double(M) * double(N) * double(K) * double(batch);
+  const double elems = double(std::max(std::max(M * K, K * N), M * N)) * double(batch);
+  return batch <= 256 && flop <= 34359738368.0 && elems <= 67108864.0;
+}
+
 inline Plan plain_config_plan(int id) {
   if (id == kPlainSmall) return {false, 1, true};
   return {id < kPlainNumSplits, kPlainSplits[id % kPlainNumSplits], false};
@@ -247,7 +258,7 @@ template <typename T>
 Plan choose_plain(pb_handle_t h, const GemmCall<T>& g, int forced = -1) {
   const bool small_ok = small::eligible(h, g.M, g.N, g.batch, g.interleaved);
   int id = forced;
-  if (id == -1 && h->plain_selector.n_layers > 0) {
+  if (id == -1 && h->plain_selector.n_layers > 0 && plain_selector_domain(g.M, g.N, g.K, g.batch)) {
     float f[8];
     plain_features(sizeof(T) == 4 ? 1 : 2, g.ta, g.tb, g.M, g.N, g.K, g.batch, g.lda, f);
     id = selector_eval(h->plain_selector, f);
@@ -400,6 +411,10 @@ pb_status_t symm_entry(pb_handle_t h, char side, char uplo, int64_t M, int64_t N
   g.M = M; g.N = N; g.K = k;
   g.alpha = alpha; g.beta = beta;
   g.c = c; g.ldc = ldc;
+  // Leading dimensions before the plan is sized: an installed selector reads
+  // them, and the run below must plan exactly the split sized here.
+  g.lda = s == 'l' ? k : ldb;
+  g.ldb = s == 'l' ? ldb : k;
   const size_t part = pb::align_up(plain_partial_bytes<T>(h, g), 256);
   void* ws = nullptr;
   PB_TRY(pb::workspace(h, part + size_t(k) * size_t(k) * sizeof(T), &ws));
@@ -558,13 +573,16 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
   // Workspace: [split-K partials][inverted blocks][W].  The partial region
   // holds the largest split of any GEMM of the walk: a narrow trailing update
   // can split K where the full-width one does not, and must not fall back to
-  // the handle workspace, which holds the blocks and W.
+  // the handle workspace, which holds the blocks and W.  The walk's GEMMs are
+  // sized here from their shapes alone, so they plan with the built-in rule
+  // (which reads nothing else) rather than an installed selector (whose
+  // features include the transposes and leading dimensions).
   size_t part = 0;
   {
     auto need = [&](int64_t m, int64_t n, int64_t kk) {
       GemmCall<T> g;
       g.M = m; g.N = n; g.K = kk;
-      if (m > 0 && n > 0) part = std::max(part, plain_partial_bytes<T>(h, g));
+      if (m > 0 && n > 0) part = std::max(part, plain_partial_bytes<T>(h, g, kRuleOnly));
     };
     PB_TRY(walk(
         [&](int64_t, int64_t bs, T) -> pb_status_t {
@@ -600,7 +618,7 @@ pb_status_t trsm_entry(pb_handle_t h, char side, char uplo, char trans, char dia
     g.ta = ta; g.tb = tb; g.M = m; g.N = n; g.K = kk;
     g.alpha = al; g.beta = be;
     g.a = A_; g.lda = la; g.b = B_; g.ldb = lb; g.c = C_; g.ldc = lc;
-    return run_plain<T>(h, g, false, partial);
+    return run_plain<T>(h, g, false, partial, kRuleOnly);
   };
   // A(r, c) of op(A): A + r + c lda ('n') or, transposed, A + c + r lda.
   auto opA = [&](int64_t r, int64_t c) { return t == 'n' ? a + r + c * lda : a + c + r * lda; };
@@ -718,6 +736,7 @@ pb_status_t cgemm_entry(pb_handle_t h, char ta, char tb, int64_t M, int64_t N, i
   gc.ta = 'n'; gc.tb = 'n';
   gc.M = 2 * M; gc.N = N; gc.K = 2 * K;
   gc.alpha = T(1); gc.beta = T(0);
+  gc.lda = 2 * M; gc.ldb = 2 * K; gc.ldc = 2 * M;  // the packed problem's (read by a selector)
   const size_t part = pb::align_up(plain_partial_bytes<T>(h, gc), 256);
   const size_t ahat_b = pb::align_up(size_t(4) * M * K * sizeof(T), 256);
   const size_t bhat_b = pb::align_up(size_t(2) * K * N * sizeof(T), 256);
@@ -853,7 +872,7 @@ pb_status_t pb_tune_time_plain(pb_handle_t h, int dtype, int id, char ta, char t
 
 int pb_tune_select_plain(pb_handle_t h, int dtype, char ta, char tb, int64_t m, int64_t n, int64_t k,
                          int64_t batch, int64_t lda) {
-  if (!h || h->plain_selector.n_layers == 0) return -1;
+  if (!h || h->plain_selector.n_layers == 0 || !plain_selector_domain(m, n, k, batch)) return -1;
   float f[8];
   plain_features(dtype, pb::norm_trans(ta), pb::norm_trans(tb), m, n, k, batch, lda, f);
   return selector_eval(h->plain_selector, f);

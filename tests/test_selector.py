@@ -196,6 +196,10 @@ def test_gpu_plain_selector_inference_and_dispatch(handle, torch_cuda):
     assert np.mean(host == gpu[:300]) >= 0.99
     # SB_Handle() installs the committed network by default
     assert np.array_equal(np.array([tuning.select_plain(handle, *s) for s in shapes[:50]]), host[:50])
+    # outside the training domain the built-in rule decides
+    for s in [(1, "n", "n", 8192, 8192, 8192, 1, 8192), (1, "n", "n", 64, 64, 64, 4096, 64),
+              (2, "n", "n", 4096, 4096, 4096, 1, 4096)]:
+        assert tuning.select_plain(h, *s) == -1
     h.close()
 
 
