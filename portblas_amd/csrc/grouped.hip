@@ -104,15 +104,15 @@ constexpr int kLG = kLK / 8;     // 16-byte groups per K-contiguous row of a chu
 constexpr int kLIt = kLT * kLG / 256;  // 16-byte loads per thread and operand
 
 template <bool KC>
-__device__ __forceinline__ void lean_load(const uint16_t* x, int64_t rs, int64_t ks, int64_t rows,
-                                          int64_t K, int64_t r0, int64_t k0, bool vec, int e,
-                                          u32x4& out) {
+__device__ __forceinline__ void lean_load(pbdev::gptr<const uint16_t> x, int64_t rs, int64_t ks,
+                                          int64_t rows, int64_t K, int64_t r0, int64_t k0, bool vec,
+                                          int e, u32x4& out) {
   // e in [0, kLT * kLG): KC -> row e / kLG, k group (e % kLG) * 8; else k e / 8,
   // row group (e % 8) * 8.
   const int64_t r = KC ? r0 + e / kLG : r0 + (e % 8) * 8;
   const int64_t k = KC ? k0 + (e % kLG) * 8 : k0 + e / 8;
   if (vec && (KC ? (r < rows && k + 8 <= K) : (k < K && r + 8 <= rows))) {
-    out = *reinterpret_cast<const u32x4*>(x + r * rs + k * ks);
+    out = *(pbdev::gptr<const u32x4>)(x + r * rs + k * ks);
     return;
   }
   // Edge or unaligned: element by element (kept rolled: it is rare, and
@@ -154,15 +154,16 @@ __device__ __forceinline__ void lean_store(uint16_t (*dst)[kLP], int e, const u3
 // One BF16 problem's operand sources, read once per tile so that the loads of
 // the next tile can be issued without waiting on its descriptor.
 struct LeanSrc {
-  const uint16_t* a;
-  const uint16_t* b;
+  pbdev::gptr<const uint16_t> a;
+  pbdev::gptr<const uint16_t> b;
   int64_t a_rs, a_ks, b_rs, b_ks, M, N, K;
   bool vec;
 };
 
 __device__ __forceinline__ LeanSrc lean_src(const DevProblem& P) {
-  return {static_cast<const uint16_t*>(P.a), static_cast<const uint16_t*>(P.b), P.a_rs, P.a_ks,
-          P.b_rs, P.b_ks, P.M, P.N, P.K, P.vec != 0};
+  return {pbdev::to_global(static_cast<const uint16_t*>(P.a)),
+          pbdev::to_global(static_cast<const uint16_t*>(P.b)),
+          P.a_rs, P.a_ks, P.b_rs, P.b_ks, P.M, P.N, P.K, P.vec != 0};
 }
 
 template <bool KCA, bool KCB>
@@ -233,18 +234,18 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
 #pragma unroll
       for (int q = 0; q < 4; ++q) {
         const int64_t m = m0 + wm + 8 * q + 4 * (lane / 32);
-        float* cp = P.c + m + n * P.ldc;
+        const pbdev::gptr<float> cp = pbdev::to_global(P.c) + m + n * P.ldc;
         if (P.c_vec && m + 3 < P.M) {
           f32x4 o;
           if (P.beta != 0.f) {
-            const f32x4 old = *reinterpret_cast<const f32x4*>(cp);
+            const f32x4 old = *(pbdev::gptr<const f32x4>)cp;
 #pragma unroll
             for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e] + P.beta * old[e];
           } else {
 #pragma unroll
             for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e];
           }
-          *reinterpret_cast<f32x4*>(cp) = o;
+          *(pbdev::gptr<f32x4>)cp = o;
         } else {
 #pragma unroll
           for (int e = 0; e < 4; ++e)
@@ -262,15 +263,22 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
 // ---------------------------------------------------------------------------
 // Ring variant of the lean kernel for problems whose M, N and K are multiples
 // of 8 with 16-byte aligned operands (every 16-byte group of 8 elements is
-// then wholly inside or wholly outside the operand).  Each workgroup walks the
-// flattened sequence of its tiles' K chunks with kRD chunks in flight in a
-// register ring, so a chunk waits for a fraction of one memory latency
-// instead of a whole one: every thread issues exactly one 16-byte load per
-// operand per chunk (lanes outside the operand read a zero block), which
-// keeps the compiler's wait counts exact, and chunks past the end of the list
-// re-read the last one (never consumed).  Two LDS buffers take alternate
-// chunks, so one barrier per chunk separates a buffer's reads from its next
-// writes.
+// then wholly inside or wholly outside the operand) and leading dimensions
+// below 2^24.  Each workgroup walks the flattened sequence of its tiles' K
+// chunks with kRD chunks in flight in a register ring, so a chunk waits for a
+// fraction of one memory latency instead of a whole one, and chunks past the
+// end of the list re-read the last one (never consumed).  Two LDS buffers
+// take alternate chunks, so one barrier per chunk separates a buffer's reads
+// from its next writes.
+//
+// The loads are raw buffer loads, one 16-byte load per thread and operand
+// per chunk, so that a chunk costs few instructions (the kernel is otherwise
+// bound by instruction issue, not by memory): each thread's byte offset
+// inside the tile's operand panel is computed once per tile (a thread whose
+// rows are outside the operand gets an offset past the buffer's range, which
+// the hardware answers with zeros), a chunk moves the buffer's base by a
+// scalar step, and in a tile's last chunk a thread whose K group is past K
+// is sent past the range too.
 // ---------------------------------------------------------------------------
 // Ring geometry: K chunk kRK (64: every row piece of a K-contiguous operand,
 // and every column piece of an M/N-contiguous one, is a whole 128-byte line;
@@ -278,64 +286,89 @@ __global__ __launch_bounds__(256) void grouped_lean_kernel(const DevProblem* __r
 // flight per workgroup.
 constexpr int kRK = 64;
 constexpr int kRD = 4;
-constexpr int kRP = kRK + 8;              // LDS row pitch ([row][k], elements)
+constexpr int kRP = kRK;                  // LDS row pitch ([row][k], elements; no pad, see ring_swz)
 constexpr int kRG = kRK / 8;              // 16-byte groups per K-contiguous row of a chunk
 constexpr int kRIt = kLT * kRG / 256;     // 16-byte loads per thread and operand
-
-__device__ __attribute__((aligned(16))) const uint32_t kZero16[4] = {0u, 0u, 0u, 0u};
-
-struct RingCursor {
-  const uint16_t* a;
-  const uint16_t* b;
-  int64_t a_rs, a_ks, b_rs, b_ks, M, N, K, m0, n0;
-  int i, kc, nk;  // tile (list index), chunk, chunks of the tile
-  int prob;
-};
-
-__device__ __forceinline__ void ring_set_tile(RingCursor& c, const DevProblem* probs, const int4* tiles, int i) {
-  const int4 t = tiles[i];
-  const DevProblem& P = probs[t.x];
-  c.a = static_cast<const uint16_t*>(P.a);
-  c.b = static_cast<const uint16_t*>(P.b);
-  c.a_rs = P.a_rs; c.a_ks = P.a_ks; c.b_rs = P.b_rs; c.b_ks = P.b_ks;
-  c.M = P.M; c.N = P.N; c.K = P.K;
-  c.m0 = int64_t(t.y) * kLT;
-  c.n0 = int64_t(t.z) * kLT;
-  c.i = i;
-  c.kc = 0;
-  c.nk = P.K > 0 ? int((P.K + kRK - 1) / kRK) : 1;
-  c.prob = t.x;
-}
+// Buffer range of every panel: in-panel offsets are below 63 * 2^24 * 2 +
+// 128 bytes (leading dimensions below 2^24), so below the range; kRFar is
+// above it, so a load at kRFar reads zeros.
+constexpr int kRRange = 0x7ffffff0;
+constexpr uint32_t kRFar = 0xfffffff0u;
+constexpr int64_t kRMaxLd = int64_t(1) << 24;
 
 // Group e of a chunk: KC -> row e / kRG, k group (e % kRG) * 8; else k
-// e / 8, row group (e % 8) * 8.  One 16-byte load (a zero block outside the
-// operand).
+// e / 8, row group (e % 8) * 8.
 template <bool KC>
-__device__ __forceinline__ u32x4 ring_group(const uint16_t* x, int64_t rs, int64_t ks, int64_t rows, int64_t K,
-                                            int64_t r0, int64_t k0, int e) {
-  const int64_t r = KC ? r0 + e / kRG : r0 + (e % 8) * 8;
-  const int64_t k = KC ? k0 + (e % kRG) * 8 : k0 + e / 8;
-  const uint16_t* p = (r < rows && k < K) ? x + r * rs + k * ks : reinterpret_cast<const uint16_t*>(kZero16);
-  return *reinterpret_cast<const u32x4*>(p);
+__device__ __forceinline__ int ring_row(int e) { return KC ? e / kRG : (e % 8) * 8; }
+template <bool KC>
+__device__ __forceinline__ int ring_k(int e) { return KC ? (e % kRG) * 8 : e / 8; }
+
+// The producer: the next chunk to load and where its groups are.
+struct RingSrc {
+  const char* a;            // operand panels at the current chunk
+  const char* b;
+  int64_t a_step, b_step;   // bytes from one chunk to the next
+  uint32_t oa[kRIt], ob[kRIt];  // this thread's byte offsets in the panels
+  int krem;                 // K left from the current chunk on
+  int kc, nk;               // chunk, chunks of the tile
+};
+
+__device__ __forceinline__ int ring_chunks(int64_t K) { return K > 0 ? int((K + kRK - 1) / kRK) : 1; }
+
+template <bool KC>
+__device__ __forceinline__ uint32_t ring_offset(int64_t rs, int64_t ks, int64_t rows_left, int e) {
+  const int r = ring_row<KC>(e), k = ring_k<KC>(e);
+  if (r >= rows_left) return kRFar;
+  return 2u * (__umul24(uint32_t(r), uint32_t(rs)) + __umul24(uint32_t(k), uint32_t(ks)));
 }
 
 template <bool KCA, bool KCB>
-__device__ __forceinline__ void ring_load(const RingCursor& c, u32x4 (&va)[kRIt], u32x4 (&vb)[kRIt]) {
-  const int64_t k0 = int64_t(c.kc) * kRK;
+__device__ __forceinline__ void ring_src_set(RingSrc& s, const DevProblem* probs, const int4* tiles, int i) {
+  const int4 t = tiles[i];
+  const DevProblem& P = probs[t.x];
+  const int64_t m0 = int64_t(t.y) * kLT, n0 = int64_t(t.z) * kLT;
+  s.a = static_cast<const char*>(P.a) + 2 * m0 * P.a_rs;
+  s.b = static_cast<const char*>(P.b) + 2 * n0 * P.b_rs;
+  s.a_step = 2 * int64_t(kRK) * P.a_ks;
+  s.b_step = 2 * int64_t(kRK) * P.b_ks;
 #pragma unroll
   for (int it = 0; it < kRIt; ++it) {
     const int e = threadIdx.x + 256 * it;
-    va[it] = ring_group<KCA>(c.a, c.a_rs, c.a_ks, c.M, c.K, c.m0, k0, e);
-    vb[it] = ring_group<KCB>(c.b, c.b_rs, c.b_ks, c.N, c.K, c.n0, k0, e);
+    s.oa[it] = ring_offset<KCA>(P.a_rs, P.a_ks, P.M - m0, e);
+    s.ob[it] = ring_offset<KCB>(P.b_rs, P.b_ks, P.N - n0, e);
+  }
+  s.krem = int(P.K);
+  s.kc = 0;
+  s.nk = ring_chunks(P.K);
+}
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t ring_rsrc(const char* base) {
+  // Raw buffer (stride 0); third dword = default data format for gfx9 parts.
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<char*>(base), (short)0, kRRange, 0x00020000);
+}
+
+template <bool KCA, bool KCB>
+__device__ __forceinline__ void ring_load(const RingSrc& s, u32x4 (&va)[kRIt], u32x4 (&vb)[kRIt]) {
+  const __amdgpu_buffer_rsrc_t ra = ring_rsrc(s.a), rb = ring_rsrc(s.b);
+#pragma unroll
+  for (int it = 0; it < kRIt; ++it) {
+    const int e = threadIdx.x + 256 * it;
+    const uint32_t oa = ring_k<KCA>(e) < s.krem ? s.oa[it] : kRFar;
+    const uint32_t ob = ring_k<KCB>(e) < s.krem ? s.ob[it] : kRFar;
+    va[it] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(ra, oa, 0, 0));
+    vb[it] = __builtin_bit_cast(u32x4, __builtin_amdgcn_raw_buffer_load_b128(rb, ob, 0, 0));
   }
 }
 
-// 16-byte chunk c of LDS row r sits at chunk c ^ ring_swz(r).  With the
-// 144-byte pitch, 16 consecutive rows at one chunk fall on 16 different
-// 16-byte slots of a 256-byte window (one swizzle value per 16-row block keeps
-// the fragment reads conflict-free), and the swizzle spreads the transposing
-// 2-byte stores (8 rows 8 apart, which the pitch alone puts on 2 slots) over 8.
-__device__ __forceinline__ int ring_swz(int r) { return (r >> 4) & (kRG - 1); }
+// 16-byte chunk c of LDS row r sits at chunk c ^ ring_swz(r).  LDS has 32
+// 4-byte banks and a row is 128 bytes, so row r's chunk c always falls on
+// the 16-byte bank slot c ^ ring_swz(r).  The 8 lanes of one cycle of a
+// fragment read (8 consecutive rows, one chunk) see r % 8 take every value
+// once, hence 8 different slots; the 32 lanes of one cycle of a transposing
+// 2-byte store (rows 8 apart, one chunk) see r / 8 take every value once,
+// hence 8 different slots again; a 16-byte store row covers all 8 chunks.
+// No access pattern of the kernel has a bank conflict.
+__device__ __forceinline__ int ring_swz(int r) { return (r ^ (r >> 3)) & (kRG - 1); }
 
 template <bool KC>
 __device__ __forceinline__ void ring_store(uint16_t (*dst)[kRP], int e, const u32x4& v) {
@@ -354,23 +387,98 @@ __device__ __forceinline__ void ring_store(uint16_t (*dst)[kRP], int e, const u3
   }
 }
 
+// Three workgroups per CU (three waves per SIMD: at most 168 registers per
+// lane, AGPRs included).
+// Epilogue staging: each wave's 32 x 32 FP32 quadrant, column-major with a
+// 144-byte pitch (36 floats), in a region of its own.
+constexpr int kRCP = 36;
+
+// Writes a wave's finished 32 x 32 quadrant of C.  The MFMA result (lane:
+// column lane % 32, rows 8 q + 4 (lane / 32) + 0..3) goes through the wave's
+// LDS region so that each store instruction writes 8 whole 128-byte column
+// pieces (8 lanes per column, 16 bytes each) instead of 32 pieces of 32
+// bytes: the store requests to the caches drop four-fold.  Lanes of one
+// 16-byte write or read fall on different banks in every cycle (pitch 36
+// floats).  C rows are multiples of 8 in this kernel, so a 4-row piece is
+// wholly inside or outside.
+__device__ __forceinline__ void ring_epilogue(const f32x16& acc, float (*stage)[kRCP], const DevProblem& P,
+                                              int64_t m_base, int64_t n_base, int lane) {
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    f32x4 v;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) v[e] = acc[4 * q + e];
+    *reinterpret_cast<f32x4*>(&stage[lane % 32][8 * q + 4 * (lane / 32)]) = v;
+  }
+  // The quadrant is read back by other lanes of the same wave: LDS serves a
+  // wave's requests in order, so only the compiler must keep the order.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int rp = (lane % 8) * 4;
+  const int64_t m = m_base + rp;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int cn = lane / 8 + 8 * j;
+    const int64_t n = n_base + cn;
+    const f32x4 v = *reinterpret_cast<const f32x4*>(&stage[cn][rp]);
+    if (n < P.N && m < P.M) {
+      const pbdev::gptr<float> cp = pbdev::to_global(P.c) + m + n * P.ldc;
+      if (P.c_vec) {
+        f32x4 o;
+        if (P.beta != 0.f) {
+          const f32x4 old = *(pbdev::gptr<const f32x4>)cp;
+#pragma unroll
+          for (int e = 0; e < 4; ++e) o[e] = P.alpha * v[e] + P.beta * old[e];
+        } else {
+#pragma unroll
+          for (int e = 0; e < 4; ++e) o[e] = P.alpha * v[e];
+        }
+        // Streaming store: C is not read again here, and keeping it out of
+        // the caches' way of the operand panels cut the launch by 11 %.
+        __builtin_nontemporal_store(o, (pbdev::gptr<f32x4>)cp);
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const float w = P.alpha * v[e];
+          cp[e] = (P.beta != 0.f) ? w + P.beta * cp[e] : w;
+        }
+      }
+    }
+  }
+  // The region is rewritten at the wave's next tile: its reads come first.
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Workgroup w runs tiles[starts[w] .. starts[w + 1]) (tile = {problem, tile
+// row, tile column, K chunks of this and every later tile of the list}), a
+// list the plan balanced by each tile's K.
 template <bool KCA, bool KCB>
-__global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __restrict__ probs,
-                                                           const int4* __restrict__ tiles,
-                                                           const int* __restrict__ starts) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))) void grouped_ring_kernel(
+    const DevProblem* __restrict__ probs, const int4* __restrict__ tiles, const int* __restrict__ starts) {
   __shared__ __attribute__((aligned(16))) uint16_t As[2][kLT][kRP];
   __shared__ __attribute__((aligned(16))) uint16_t Bs[2][kLT][kRP];
+  __shared__ __attribute__((aligned(16))) float Cs[4][32][kRCP];
   const int begin = starts[blockIdx.x], end = starts[blockIdx.x + 1];
   if (begin >= end) return;
+  const int total = tiles[begin].w;  // K chunks of the whole list
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = (wave & 1) * 32, wn = (wave >> 1) * 32;
-  // Producer (kRD chunks ahead) and consumer cursors.
-  RingCursor pc, cc;
-  ring_set_tile(pc, probs, tiles, begin);
-  cc = pc;
+  // Producer (kRD chunks ahead): pi is its tile; consumer: ci, ckc, cnk.
+  RingSrc pc;
+  int pi = begin;
+  ring_src_set<KCA, KCB>(pc, probs, tiles, pi);
+  int ci = begin, ckc = 0, cnk = pc.nk;
   auto advance_producer = [&]() {
-    if (pc.kc + 1 < pc.nk) ++pc.kc;
-    else if (pc.i + 1 < end) ring_set_tile(pc, probs, tiles, pc.i + 1);
+    if (pc.kc + 1 < pc.nk) {
+      ++pc.kc;
+      pc.a += pc.a_step;
+      pc.b += pc.b_step;
+      pc.krem -= kRK;
+    } else if (pi + 1 < end) {
+      ring_src_set<KCA, KCB>(pc, probs, tiles, ++pi);
+    }
     // else: stay on the last chunk (re-read, never consumed)
   };
   u32x4 va[kRD][kRIt], vb[kRD][kRIt];
@@ -382,10 +490,11 @@ __global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __r
   f32x16 acc;
 #pragma unroll
   for (int e = 0; e < 16; ++e) acc[e] = 0.f;
-  bool more = true;
   // One chunk: stage slot S into LDS buffer S % 2, refill the slot kRD
   // chunks ahead, run the chunk's MFMAs, and finish the tile after its last
-  // chunk.  kRD is even, so the buffer parity follows the slot.
+  // chunk.  kRD is even, so the buffer parity follows the slot.  The
+  // producer moves to its next chunk after the MFMAs, so the descriptor
+  // loads of a new tile do not hold up the fragment reads' waits.
   auto step = [&](auto slot) {
     constexpr int S = decltype(slot)::value;
     constexpr int BUF = S & 1;
@@ -397,7 +506,6 @@ __global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __r
     }
     __syncthreads();
     ring_load<KCA, KCB>(pc, va[S], vb[S]);
-    advance_producer();
 #pragma unroll
     for (int kk = 0; kk < kRK; kk += 16) {
       bf16x8 a, b;
@@ -406,59 +514,42 @@ __global__ __launch_bounds__(256) void grouped_ring_kernel(const DevProblem* __r
       __builtin_memcpy(&b, &Bs[BUF][rb][(c ^ ring_swz(rb)) * 8], 16);
       acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, acc, 0, 0, 0);
     }
-    if (cc.kc + 1 < cc.nk) {
-      ++cc.kc;
+    advance_producer();
+    if (ckc + 1 < cnk) {
+      ++ckc;
       return;
     }
-    // Epilogue: lane holds column wn + lane % 32, rows wm + 8 (e / 4) +
-    // 4 (lane / 32) + e % 4.
-    const DevProblem& P = probs[cc.prob];
-    const int64_t n = cc.n0 + wn + lane % 32;
-    if (n < P.N) {
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int64_t m = cc.m0 + wm + 8 * q + 4 * (lane / 32);
-        float* cp = P.c + m + n * P.ldc;
-        if (P.c_vec && m + 3 < P.M) {
-          f32x4 o;
-          if (P.beta != 0.f) {
-            const f32x4 old = *reinterpret_cast<const f32x4*>(cp);
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e] + P.beta * old[e];
-          } else {
-#pragma unroll
-            for (int e = 0; e < 4; ++e) o[e] = P.alpha * acc[4 * q + e];
-          }
-          *reinterpret_cast<f32x4*>(cp) = o;
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (m + e < P.M) {
-              const float v = P.alpha * acc[4 * q + e];
-              cp[e] = (P.beta != 0.f) ? v + P.beta * cp[e] : v;
-            }
-        }
-      }
-    }
+    const int4 t = tiles[ci];
+    ring_epilogue(acc, Cs[wave], probs[t.x], int64_t(t.y) * kLT + wm, int64_t(t.z) * kLT + wn, lane);
 #pragma unroll
     for (int e = 0; e < 16; ++e) acc[e] = 0.f;
-    if (cc.i + 1 < end) ring_set_tile(cc, probs, tiles, cc.i + 1);
-    else more = false;
+    if (ci + 1 < end) {
+      ++ci;
+      ckc = 0;
+      cnk = ring_chunks(probs[tiles[ci].x].K);
+    }
   };
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
   using I2 = std::integral_constant<int, 2>;
   using I3 = std::integral_constant<int, 3>;
   static_assert(kRD == 4, "the step sequence below is written for four slots");
-  while (more) {
+  // Whole rounds of four chunks in a loop with its only exit at the round's
+  // end, then the last 0-3 chunks.  The compiler's wait counts at slot 0
+  // merge every edge into the loop head: an early exit routed through the
+  // head (as the structurizer does with breaks) would make them assume a
+  // slot 0 refilled one step ago and wait for nearly every load in flight.
+  int c = 0;
+  for (; c + kRD <= total; c += kRD) {
     step(I0{});
-    if (!more) break;
     step(I1{});
-    if (!more) break;
     step(I2{});
-    if (!more) break;
     step(I3{});
   }
+  const int rem = total - c;
+  if (rem > 0) step(I0{});
+  if (rem > 1) step(I1{});
+  if (rem > 2) step(I2{});
   // The ring's last loads (re-reads, never consumed) land before the wave
   // ends and its registers go to another wave.
   __builtin_amdgcn_s_waitcnt(0);
@@ -509,18 +600,22 @@ void launch_lean(pb_handle_t h, const Launch& L) {
   }
 }
 
-// Balances the tiles of one BF16 launch over `grid` workgroups, XCD-aware:
-// workgroup w runs on XCD w % 8, and each of the 8 XCDs has its own L2, so
-// every problem goes whole to one XCD (longest first, to the least-loaded
-// XCD) and its operand panels are fetched into one L2 only.  A problem worth
-// more than a sixteenth of the launch is spread over all XCDs instead.
-// Within an XCD the tiles go longest first to the least-loaded workgroup
-// (cost of a tile: K chunks + 1 for the epilogue).  Returns per-workgroup
-// lists, concatenated, and their offsets.
+// The tiles of one BF16 launch, XCD by XCD.  Workgroup w runs on XCD w % 8,
+// and each of the 8 XCDs has its own L2, so every problem goes whole to one
+// XCD (longest first, to the least-loaded XCD) and its operand panels are
+// fetched into one L2 only.  A problem worth more than a sixteenth of the
+// launch is spread over all XCDs, tile by tile, instead.  Cost of a tile: K
+// chunks + 1 for the epilogue.  Each XCD's tiles come longest first, a
+// problem's tiles together (the XCD's workgroups work through one problem's
+// panels at a time).
 constexpr int kXcds = 8;
 
-void balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, std::vector<int4>& list,
-                   std::vector<int>& starts) {
+struct TileItem {
+  int64_t cost;
+  int4 t;  // problem, tile row, tile column
+};
+
+std::vector<std::vector<TileItem>> xcd_tiles(const std::vector<DevProblem>& v, int xcds, int kchunk) {
   struct Item { int64_t cost; int prob; };
   std::vector<Item> probs_by_cost;
   int64_t total = 0;
@@ -532,53 +627,57 @@ void balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, std::
     return x.cost * int64_t(v[x.prob].tiles_m) * v[x.prob].tiles_n >
            y.cost * int64_t(v[y.prob].tiles_m) * v[y.prob].tiles_n;
   });
-  const int xcds = grid >= kXcds ? kXcds : 1;
-  // Problems to XCDs (or "spread" for the very large ones).
   std::vector<int64_t> xload(xcds, 0);
-  std::vector<std::vector<Item>> by_xcd(xcds);
-  std::vector<Item> spread;
+  std::vector<std::vector<TileItem>> out(xcds);
   for (const Item& it : probs_by_cost) {
-    const int64_t c = it.cost * int64_t(v[it.prob].tiles_m) * v[it.prob].tiles_n;
-    if (xcds == 1 || c * 16 > total) {
-      spread.push_back(it);
+    const DevProblem& d = v[it.prob];
+    const int64_t c = it.cost * int64_t(d.tiles_m) * d.tiles_n;
+    if (xcds > 1 && c * 16 > total) {
+      for (int32_t tm = 0; tm < d.tiles_m; ++tm)
+        for (int32_t tn = 0; tn < d.tiles_n; ++tn) {
+          const int x = int(std::min_element(xload.begin(), xload.end()) - xload.begin());
+          xload[x] += it.cost;
+          out[x].push_back({it.cost, int4{it.prob, tm, tn, 0}});
+        }
       continue;
     }
     const int x = int(std::min_element(xload.begin(), xload.end()) - xload.begin());
     xload[x] += c;
-    by_xcd[x].push_back(it);
+    for (int32_t tm = 0; tm < d.tiles_m; ++tm)
+      for (int32_t tn = 0; tn < d.tiles_n; ++tn) out[x].push_back({it.cost, int4{it.prob, tm, tn, 0}});
   }
-  // Tiles to workgroups.  One heap per XCD over its workgroups (w % xcds == x).
+  for (auto& q : out)
+    std::stable_sort(q.begin(), q.end(), [](const TileItem& p1, const TileItem& p2) { return p1.cost > p2.cost; });
+  return out;
+}
+
+// Static schedule: each XCD's tiles, in order, to the least-loaded of its
+// workgroups (w % xcds == x).  Returns per-workgroup lists, concatenated, and
+// their offsets; each tile's .w holds the K chunks (of kchunk) of that tile
+// and of every later tile of its list (the ring kernel's chunk count), or
+// false if a list has 2^31 chunks or more.
+bool balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, std::vector<int4>& list,
+                   std::vector<int>& starts) {
+  const int xcds = grid >= kXcds ? kXcds : 1;
+  const auto by_xcd = xcd_tiles(v, xcds, kchunk);
   using Load = std::pair<int64_t, int>;  // (assigned cost, workgroup)
   std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> heaps(xcds);
   for (int w = 0; w < grid; ++w) heaps[w % xcds].push({0, w});
   std::vector<std::vector<int4>> per(grid);
-  auto place = [&](int x, const Item& it, int32_t tm, int32_t tn) {
-    Load l = heaps[x].top();
-    heaps[x].pop();
-    per[l.second].push_back(int4{it.prob, tm, tn, 0});
-    l.first += it.cost;
-    heaps[x].push(l);
-  };
-  // Spread problems first (largest), round-robin over XCDs by least load.
-  for (const Item& it : spread) {
-    const DevProblem& d = v[it.prob];
-    for (int32_t tm = 0; tm < d.tiles_m; ++tm)
-      for (int32_t tn = 0; tn < d.tiles_n; ++tn) {
-        int x = 0;
-        for (int y = 1; y < xcds; ++y)
-          if (heaps[y].top().first < heaps[x].top().first) x = y;
-        place(x, it, tm, tn);
-      }
-  }
-  // Within an XCD: longest tiles first (a problem's tiles stay together, so
-  // the XCD's workgroups work through one problem's panels at a time).
-  for (int x = 0; x < xcds; ++x) {
-    std::stable_sort(by_xcd[x].begin(), by_xcd[x].end(),
-                     [](const Item& p1, const Item& p2) { return p1.cost > p2.cost; });
-    for (const Item& it : by_xcd[x]) {
-      const DevProblem& d = v[it.prob];
-      for (int32_t tm = 0; tm < d.tiles_m; ++tm)
-        for (int32_t tn = 0; tn < d.tiles_n; ++tn) place(x, it, tm, tn);
+  for (int x = 0; x < xcds; ++x)
+    for (const TileItem& it : by_xcd[x]) {
+      Load l = heaps[x].top();
+      heaps[x].pop();
+      per[l.second].push_back(it.t);
+      l.first += it.cost;
+      heaps[x].push(l);
+    }
+  for (auto& q : per) {
+    int64_t suffix = 0;
+    for (size_t i = q.size(); i-- > 0;) {
+      suffix += std::max<int64_t>(1, pb::ceil_div(v[q[i].x].K, kchunk));
+      if (suffix > INT32_MAX) return false;
+      q[i].w = int(suffix);
     }
   }
   list.clear();
@@ -587,6 +686,7 @@ void balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, std::
     list.insert(list.end(), q.begin(), q.end());
     starts.push_back(int(list.size()));
   }
+  return true;
 }
 
 template <typename T, typename Shape>
@@ -654,7 +754,8 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     d.tiles_n = int32_t(pb::ceil_div(q.n, bn));
     d.vec = aligned16(q.a) && aligned16(q.b) && q.lda % vw == 0 && q.ldb % vw == 0;
     d.c_vec = aligned16(q.c) && q.ldc % 4 == 0;
-    const bool ring = bf16 && d.vec && q.m % 8 == 0 && q.n % 8 == 0 && q.k % 8 == 0;
+    const bool ring = bf16 && d.vec && q.m % 8 == 0 && q.n % 8 == 0 && q.k % 8 == 0 &&
+                      q.lda < kRMaxLd && q.ldb < kRMaxLd && q.k <= 0x7fffffffLL - kRK;
     by_layout[2 * (ta == 't') + (tb == 'n') + (ring ? 4 : 0)].push_back(d);
   }
   auto upload = [&](const void* src, size_t bytes, void** dst) -> pb_status_t {
@@ -718,7 +819,10 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
       L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, lean_per_cu(lay) * h->num_cus)));
       std::vector<int4> list;
       std::vector<int> starts;
-      balance_tiles(v, L.grid, L.ring ? kRK : kLK, list, starts);
+      if (!balance_tiles(v, L.grid, L.ring ? kRK : kLK, list, starts)) {
+        destroy();
+        return pb::set_error(PB_STATUS_INVALID_VALUE, "grouped BF16 GEMM: 2^31 K chunks or more on one workgroup");
+      }
       void* dl = nullptr;
       void* ds = nullptr;
       PB_TRY(upload(list.data(), list.size() * sizeof(int4), &dl));

@@ -158,5 +158,19 @@ __device__ __forceinline__ V ld_stream(const V* p) {
   return __builtin_nontemporal_load(p);
 }
 
+// Pointers that a kernel reads out of a descriptor in memory are generic to
+// the compiler, so their accesses become flat instructions.  A flat access
+// counts against both the vector-memory and the LDS/scalar wait counters, and
+// while one is outstanding every wait the compiler inserts drains ALL pending
+// memory operations: loads issued several steps ahead are then waited for at
+// once.  Such pointers are cast to the global address space before use.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+
+template <typename T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+
 }  // namespace pbdev
 #endif
