@@ -651,25 +651,40 @@ std::vector<std::vector<TileItem>> xcd_tiles(const std::vector<DevProblem>& v, i
   return out;
 }
 
-// Static schedule: each XCD's tiles, in order, to the least-loaded of its
-// workgroups (w % xcds == x).  Returns per-workgroup lists, concatenated, and
-// their offsets; each tile's .w holds the K chunks (of kchunk) of that tile
-// and of every later tile of its list (the ring kernel's chunk count), or
-// false if a list has 2^31 chunks or more.
-bool balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, std::vector<int4>& list,
-                   std::vector<int>& starts) {
+// Relative speed of the three workgroups that share a CU when the ring
+// kernel's grid holds exactly three per CU.  The dispatcher fills every CU
+// once before it starts the next round, so workgroup w is the (w / CUs)-th
+// on its CU, and the SIMDs' issue arbitration favours older waves: with
+// equal work the three ended at 71, 84 and 98 us on average (512-problem
+// set, MI355X; within each rank the ends lay within 4 us), i.e. they run at
+// 1 : 0.84 : 0.72.  Other grids are balanced as if all ran alike.
+constexpr double kRankSpeed[3] = {1.0, 0.78, 0.61};
+
+// Static schedule: each XCD's tiles, in order, to the one of its workgroups
+// (w % xcds == x) that would finish its list first (assigned cost over its
+// speed).  Returns per-workgroup lists, concatenated, and their offsets;
+// each tile's .w holds the K chunks (of kchunk) of that tile and of every
+// later tile of its list (the ring kernel's chunk count), or false if a list
+// has 2^31 chunks or more.  `per_cu`: workgroups per CU in the grid, `cus`:
+// the device's CUs (rank speeds apply when grid == 3 * cus).
+bool balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, int per_cu, int cus,
+                   std::vector<int4>& list, std::vector<int>& starts) {
   const int xcds = grid >= kXcds ? kXcds : 1;
   const auto by_xcd = xcd_tiles(v, xcds, kchunk);
-  using Load = std::pair<int64_t, int>;  // (assigned cost, workgroup)
+  const bool ranked = per_cu == 3 && grid == 3 * cus;
+  std::vector<double> speed(grid, 1.0);
+  if (ranked)
+    for (int w = 0; w < grid; ++w) speed[w] = kRankSpeed[w / cus];
+  using Load = std::pair<double, int>;  // (finish time of the assigned cost, workgroup)
   std::vector<std::priority_queue<Load, std::vector<Load>, std::greater<Load>>> heaps(xcds);
-  for (int w = 0; w < grid; ++w) heaps[w % xcds].push({0, w});
+  for (int w = 0; w < grid; ++w) heaps[w % xcds].push({0.0, w});
   std::vector<std::vector<int4>> per(grid);
   for (int x = 0; x < xcds; ++x)
     for (const TileItem& it : by_xcd[x]) {
       Load l = heaps[x].top();
       heaps[x].pop();
       per[l.second].push_back(it.t);
-      l.first += it.cost;
+      l.first += double(it.cost) / speed[l.second];
       heaps[x].push(l);
     }
   for (auto& q : per) {
@@ -816,10 +831,11 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     PB_TRY(upload(v.data(), v.size() * sizeof(DevProblem), &dev));
     L.dev = static_cast<DevProblem*>(dev);
     if (bf16) {
-      L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, lean_per_cu(lay) * h->num_cus)));
+      const int per_cu = int(lean_per_cu(lay));
+      L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, int64_t(per_cu) * h->num_cus)));
       std::vector<int4> list;
       std::vector<int> starts;
-      if (!balance_tiles(v, L.grid, L.ring ? kRK : kLK, list, starts)) {
+      if (!balance_tiles(v, L.grid, L.ring ? kRK : kLK, L.ring ? per_cu : 0, h->num_cus, list, starts)) {
         destroy();
         return pb::set_error(PB_STATUS_INVALID_VALUE, "grouped BF16 GEMM: 2^31 K chunks or more on one workgroup");
       }
