@@ -74,6 +74,9 @@ def main(argv=None):
     ap.add_argument("--train-files", nargs="*", default=[],
                     help="timing datasets of training shapes (file i uses seed + i)")
     ap.add_argument("--heldout-file", default=None)
+    ap.add_argument("--anchors", default=None,
+                    help="plain family: timing file of the anchor shapes (timed and written if missing)")
+    ap.add_argument("--anchor-repeat", type=int, default=8, help="copies of the anchor shapes in the training set")
     ap.add_argument("--collect-only", action="store_true")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
@@ -199,6 +202,21 @@ def _plain_set(tuning, tuner_box, n, seed, max_flop, reps, path, progress_every)
     return X, T, shapes
 
 
+def _plain_anchor_set(tuning, tuner_box, reps, path):
+    """The anchor shapes (tuning.anchor_plain_shapes), timed once and kept in `path`."""
+    if Path(path).exists():
+        z = np.load(path, allow_pickle=False)
+        print(f"[tune] loaded {path}: {len(z['X'])} anchor shapes", flush=True)
+        return z["X"], z["T"], [_dec_plain(r) for r in z["shapes"]]
+    shapes = tuning.anchor_plain_shapes()
+    t0 = time.time()
+    X, T = tuning.collect_plain(tuner_box(), shapes, progress=max(1, len(shapes) // 4), reps=reps)
+    print(f"[tune] timed {len(shapes)} anchor shapes x {T.shape[1]} configs in {time.time() - t0:.0f} s", flush=True)
+    Path(path).parent.mkdir(parents=True, exist_ok=True)
+    np.savez(path, X=X, T=T, shapes=np.array([_enc_plain(s) for s in shapes], dtype=np.int64))
+    return X, T, shapes
+
+
 def _enc_plain(s):
     return [s[0], ord(s[1]), ord(s[2]), *s[3:]]
 
@@ -231,6 +249,10 @@ def main_plain(args):
                         max(1, args.train // 10)) for i, f in enumerate(train_files)]
     Xtr = np.concatenate([p[0] for p in parts])
     Ttr = np.concatenate([p[1] for p in parts])
+    if args.anchors:
+        Xa, Ta, _ = _plain_anchor_set(tuning, tuner_box, args.reps, args.anchors)
+        Xtr = np.concatenate([Xtr] + [Xa] * args.anchor_repeat)
+        Ttr = np.concatenate([Ttr] + [Ta] * args.anchor_repeat)
     Xte, Tte, te_shapes = _plain_set(tuning, tuner_box, args.heldout, args.seed + 1000, args.max_flop, args.reps,
                                      args.heldout_file, max(1, args.heldout // 5))
     if args.collect_only:

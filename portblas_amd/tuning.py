@@ -309,6 +309,28 @@ def sample_plain_shapes(n: int, seed: int, lo: int = 16, hi: int = 16384, max_ba
     return out
 
 
+def anchor_plain_shapes(max_flop: float = 2.0 ** 35, max_elems: int = 2 ** 26):
+    """Structured shapes added to the random training set: square and
+    K-stretched GEMMs of 256 .. 2048 (powers of two and 1.5 x powers of two),
+    both element types, all four transposes, batch 1, packed leading
+    dimensions; and 64^3 / 32^3 batches of 64 .. 256.  Random log-uniform
+    shapes rarely land on exact multiples of the tile edge and the CU count,
+    where the best plan changes abruptly (1024^2 is exactly one 64 x 64 tile
+    per CU on 256 CUs), and which the benchmark configurations sit on."""
+    out = []
+    for dt in PLAIN_DTYPES:
+        for ta in "nt":
+            for tb in "nt":
+                for s in (256, 384, 512, 768, 1024, 1536, 2048):
+                    for K in (s // 2, s, 2 * s):
+                        if 2.0 * s * s * K > max_flop or max(s * K, K * s, s * s) > max_elems:
+                            continue
+                        out.append((dt, ta, tb, s, s, K, 1, s if ta == "n" else K))
+                for e, b in ((64, 64), (64, 128), (64, 256), (32, 256)):
+                    out.append((dt, ta, tb, e, e, e, b, e))
+    return out
+
+
 class PlainAutotuner:
     """Exhaustive timing of every plain configuration for FP32 and FP64
     shapes, plus the built-in rule (the last column).  Configurations whose

@@ -165,6 +165,33 @@ def test_plain_selector_within_5pct_of_exhaustive_and_beats_rule():
     assert set(np.unique(d["X"][:, 5])) == {1.0, 2.0} and np.mean(d["X"][:, 7] > 0) > 0.2
 
 
+PLAIN_ANCHORS = ROOT / "tests" / "golden" / "selector_plain_anchors_mi355x.npz"
+
+
+def test_plain_selector_on_anchor_shapes():
+    """The structured shapes the network was also trained on (squares and
+    K-stretched squares of 256 .. 2048, small-problem batches; timed
+    exhaustively on MI355X): its picks stay within 5% of the fastest config
+    at every one of them, and at the reference's CPU-runnable configuration
+    (1024^3 FP32 'n','n', one 64 x 64 tile per CU) it picks the fastest."""
+    from portblas_amd import tuning
+
+    net, _ = _load_plain()
+    a = np.load(PLAIN_ANCHORS, allow_pickle=False)
+    X, T, S = a["X"], a["T"].astype(np.float64), a["shapes"]
+    assert len(S) == len(tuning.anchor_plain_shapes())
+    for row, x in zip(S, X):
+        dt, ta, tb = int(row[0]), chr(int(row[1])), chr(int(row[2]))
+        np.testing.assert_allclose(tuning.plain_features(dt, ta, tb, *(int(v) for v in row[3:])), x, rtol=1e-6)
+    nconf = T.shape[1] - 1
+    chosen = _oracle_choice(net, X)
+    best = T[:, :nconf].min(axis=1)
+    ratio = _plain_times(chosen, T) / best
+    assert ratio.max() <= 1.05, (ratio.max(), S[ratio.argmax()])
+    cfg1 = [i for i, r in enumerate(S) if list(r) == [1, ord("n"), ord("n"), 1024, 1024, 1024, 1, 1024]]
+    assert len(cfg1) == 1 and chosen[cfg1[0]] == int(np.argmin(T[cfg1[0], :nconf]))
+
+
 def test_plain_fixture_is_consistent():
     from portblas_amd import tuning
 
