@@ -403,6 +403,19 @@ def main():
         nbytes = 3.0 * bs * s * s * 4
         res["strided_batched_4096x64"] = {"TFLOP/s": 2.0 * bs * s ** 3 / t / 1e9, "GB/s": nbytes / t / 1e6,
                                           "frac_hbm": nbytes / t / 1e6 / HBM_PEAK_GBS, "us": 1e3 * t}
+        # the same call captured in a graph on a side stream and replayed
+        import ctypes
+        L = pb.load()
+        gs = torch.cuda.Stream()
+        hs = pb.SB_Handle(gs)
+        gph = ctypes.c_void_p()
+        with torch.cuda.stream(gs):
+            pb.check(L.pb_graph_capture_begin(hs.handle))
+            pb._gemm_strided_batched(hs, "n", "n", s, s, s, 1.0, Ab, s, s * s, Bb, s, s * s, 0.0, Cb, s, s * s, bs)
+            pb.check(L.pb_graph_capture_end(hs.handle, ctypes.byref(gph)))
+            tg = timer(lambda: pb.check(L.pb_graph_launch(hs.handle, gph)), reps=10, before=flush)
+        pb.check(L.pb_graph_destroy(gph))
+        res["strided_batched_4096x64"].update({"graph_us": 1e3 * tg, "graph_GB/s": nbytes / tg / 1e6})
         log(f"strided batched 4096x64^3: {res['strided_batched_4096x64']}")
         del Ab, Bb, Cb, flushbuf
         res["grouped_512_bf16"] = bench_grouped(torch, pb, h, timer)
