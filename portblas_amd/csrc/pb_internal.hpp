@@ -65,6 +65,15 @@ struct pb_selector_state {
   std::vector<int> sizes;
   std::vector<std::vector<float>> w, b;
   std::vector<float> mean, stdv;
+  // Recent decisions, direct-mapped by the feature vector (a repeated shape
+  // skips the network); installing a network replaces the whole state.
+  struct Decision {
+    bool valid = false;
+    float f[8];
+    int choice;
+  };
+  static constexpr int kCacheSlots = 64;
+  mutable Decision cache[kCacheSlots];
 };
 
 // One workspace allocation.  The handle holds the current one; every graph
