@@ -6,10 +6,12 @@
 // The plan uploads one descriptor per problem.
 //  BF16: the plan also balances every 64 x 64 tile over a persistent grid
 //    (as many workgroups per CU as are resident) by its K, longest first,
-//    each problem on one XCD, and uploads each
-//    workgroup's tile list; the lean kernel walks its list with the loads of
-//    the next chunk or tile in flight during the current one's MFMAs.  No
-//    atomics, no search, nothing to reset: the launch replays in a graph.
+//    each problem on one XCD, and uploads each workgroup's tile list.  The
+//    ring kernel (problems with dimensions multiple of 8, aligned operands)
+//    walks its list with four K chunks in flight across tile boundaries; the
+//    lean kernel (the rest) with the next chunk or tile in flight during the
+//    current one's MFMAs.  No atomics, no search, nothing to reset: the
+//    launch replays in a graph.
 //  FP32: a persistent grid pulls tile indices from a device-scope work
 //    queue, finds the owning problem by binary search over the prefix sums
 //    of the tile counts, and runs the MFMA tile engine on that tile.  The
