@@ -13,11 +13,14 @@
 //       folded across waves in LDS; with several column chunks, each writes
 //       its partials to the workspace and a fold kernel sums them in a fixed
 //       order and applies the epilogue.
-//  't'  a workgroup owns a 2048-row chunk and a group of columns; each lane
-//       keeps its 32 elements of x in registers for the whole workgroup, each
-//       wave walks its columns, one column (8 KB, 8 16-byte loads per lane)
-//       at a time, and reduces across the wave with shuffles.  Row chunks are
-//       folded like the 'n' column chunks.
+//  't'  a workgroup owns a group of columns; each lane keeps its 32 elements
+//       of a 2048-row chunk of x in registers, each wave walks its columns,
+//       one column chunk (8 KB, 8 16-byte loads per lane) at a time, and
+//       reduces across the wave with shuffles.  With enough columns to fill
+//       the GPU, a workgroup walks every row chunk itself and adds each
+//       column's chunk sums in chunk order (one launch); otherwise the row
+//       chunks are spread over the grid and folded like the 'n' column
+//       chunks, in the same order.
 #include "pb_internal.hpp"
 
 namespace {
@@ -228,6 +231,79 @@ __global__ __launch_bounds__(kBlock) void gemv_t_kernel(
   }
 }
 
+// 't' without row chunks in the grid: grid (col_groups), each workgroup walks
+// every 2048-row chunk of its columns (at most kTCols per wave), reloading
+// the chunk's x slice (cached) and adding each column's chunk sum to a
+// running total in chunk order: the same sums, in the same order, as the
+// partials and the fold launch of gemv_t_kernel, without the partials and the
+// second launch.
+constexpr int kTCols = 4;  // columns a wave keeps a running sum for
+
+template <typename T, bool VEC>
+__global__ __launch_bounds__(kBlock) void gemv_t_full_kernel(
+    int64_t M, int64_t N, T alpha, const T* __restrict__ A, int64_t lda,
+    const T* __restrict__ x, int64_t incx, T beta, T* y, int64_t incy,
+    int64_t cols_per_group) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  const int64_t j0 = int64_t(blockIdx.x) * cols_per_group;
+  int64_t j1 = j0 + cols_per_group;
+  if (j1 > N) j1 = N;
+  T colsum[kTCols];
+#pragma unroll
+  for (int c = 0; c < kTCols; ++c) colsum[c] = T(0);
+  for (int64_t r0 = 0; r0 < M; r0 += kTRows) {
+    const bool full = r0 + kTRows <= M;
+    T xr[kTQ][4];
+#pragma unroll
+    for (int q = 0; q < kTQ; ++q) {
+      if (VEC && full) {
+        Four<T> v = load4<T, false>(x + r0 + 256 * q + 4 * lane);
+#pragma unroll
+        for (int e = 0; e < 4; ++e) xr[q][e] = v.v[e];
+      } else {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int64_t r = VEC ? r0 + 256 * q + 4 * lane + e : r0 + 64 * (4 * q + e) + lane;
+          xr[q][e] = (r < M) ? x[pbdev::vidx(r, M, incx)] : T(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < kTCols; ++c) {
+      const int64_t j = j0 + wave + int64_t(kWaves) * c;
+      if (j >= j1) break;
+      const T* col = A + j * lda;
+      T acc = T(0);
+      if (VEC && full) {
+        Four<T> a[kTQ];
+#pragma unroll
+        for (int q = 0; q < kTQ; ++q) a[q] = load4(col + r0 + 256 * q + 4 * lane);
+#pragma unroll
+        for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) acc += a[q].v[e] * xr[q][e];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kTQ; ++q)
+#pragma unroll
+          for (int e = 0; e < 4; ++e) {
+            const int64_t r = VEC ? r0 + 256 * q + 4 * lane + e : r0 + 64 * (4 * q + e) + lane;
+            if (r < M) acc += pbdev::ld_stream(col + r) * xr[q][e];
+          }
+      }
+      colsum[c] += pbdev::wave_sum(acc);
+    }
+  }
+  if (lane == 0) {
+#pragma unroll
+    for (int c = 0; c < kTCols; ++c) {
+      const int64_t j = j0 + wave + int64_t(kWaves) * c;
+      if (j < j1) store_y(y, pbdev::vidx(j, N, incy), alpha, beta, colsum[c]);
+    }
+  }
+}
+
 // y <- beta * y for the degenerate shapes (an empty contraction).
 template <typename T>
 __global__ void scale_y_kernel(int64_t n, T beta, T* y, int64_t incy) {
@@ -301,6 +377,28 @@ pb_status_t gemv_launch(pb_handle_t h, char trans, int64_t M, int64_t N, T alpha
   } else {
     const bool vec = avec && incx == 1 && aligned16(x);
     const int64_t chunks = pb::ceil_div(M, kTRows);
+    // Enough columns for `target` workgroups of at most kTCols columns per
+    // wave: no row chunks in the grid, no partials, no fold launch.  (A
+    // workgroup then walks every row chunk; beyond 32 chunks, or with too few
+    // columns to fill the GPU, the chunked grid below spreads the rows.)
+    // Twice `target` workgroups (8 per CU; one column per wave at 8192 x
+    // 8192): 44.5-45.9 us for the whole call, against 48.0-49.8 us with 4 per
+    // CU and 47.7-48.9 us for the chunked grid with its fold launch.
+    if (chunks > 1 && chunks <= 32 && N >= int64_t(target) * kWaves) {
+      int64_t cpg = pb::ceil_div(N, 2 * int64_t(target));
+      cpg = pb::ceil_div(cpg, kWaves) * kWaves;
+      if (cpg > int64_t(kTCols) * kWaves) cpg = int64_t(kTCols) * kWaves;
+      const int64_t groups = pb::ceil_div(N, cpg);
+      PB_REQUIRE(groups <= 0x7fffffffLL, "N too large");
+      if (vec)
+        hipLaunchKernelGGL((gemv_t_full_kernel<T, true>), dim3(unsigned(groups)), dim3(kBlock), 0, h->stream,
+                           M, N, alpha, A, lda, x, incx, beta, y, incy, cpg);
+      else
+        hipLaunchKernelGGL((gemv_t_full_kernel<T, false>), dim3(unsigned(groups)), dim3(kBlock), 0, h->stream,
+                           M, N, alpha, A, lda, x, incx, beta, y, incy, cpg);
+      PB_CHECK_LAUNCH();
+      return PB_STATUS_SUCCESS;
+    }
     PB_REQUIRE(chunks <= 65535, "M too large for the transposed GEMV (max %d rows)",
                65535 * kTRows);
     int64_t groups = pb::ceil_div(target, chunks);

@@ -102,6 +102,19 @@ def test_gemv_t_row_chunks(handle, torch_cuda):
         _gemv_check(handle, torch_cuda, "t", m, 130, 1.25, 0.5, m, 1, 1, seed=m, dtype=dt)
 
 
+@pytest.mark.parametrize("m,n,lda,incx,incy,alpha,beta,offset,dt", [
+    (5000, 4100, 5000, 1, 1, 1.25, 0.5, 0, np.float32),   # partial last row chunk and column group
+    (8192, 4096, 8200, 2, -1, -1.0, 0.0, 0, np.float32),  # padded lda, strided x, reversed y
+    (6145, 4099, 6147, 1, 3, 1.0, 2.0, 1, np.float32),    # unaligned A and x: the scalar path
+    (4100, 4104, 4100, 1, 1, 0.5, 0.25, 0, np.float64),
+])
+def test_gemv_t_wide(handle, torch_cuda, m, n, lda, incx, incy, alpha, beta, offset, dt):
+    """'t' with at least 4096 columns and several 2048-row chunks: each
+    workgroup walks all row chunks of its columns (no partials, no fold
+    launch); every output entry against the FP64 oracle."""
+    _gemv_check(handle, torch_cuda, "t", m, n, alpha, beta, lda, incx, incy, seed=m + n, offset=offset, dtype=dt)
+
+
 @pytest.mark.parametrize("m,n,lda,incx,incy,alpha,beta,poison", [
     (8224, 600, 8224, 1, 1, 1.5, 0.5, False),    # a partial last group of tiles, beta reads y
     (8192, 1000, 8200, 2, -1, -1.0, 0.0, True),  # padded lda, strided x, reversed y, y never read
