@@ -115,6 +115,19 @@ def test_gemv_t_wide(handle, torch_cuda, m, n, lda, incx, incy, alpha, beta, off
     _gemv_check(handle, torch_cuda, "t", m, n, alpha, beta, lda, incx, incy, seed=m + n, offset=offset, dtype=dt)
 
 
+@pytest.mark.parametrize("m,n,lda,incx,incy,alpha,beta,offset,dt", [
+    (8200, 513, 8200, 1, 1, 1.25, 0.5, 0, np.float32),    # partial last 32-row block
+    (8192, 700, 8196, 3, -2, -1.0, 0.0, 0, np.float32),   # padded lda, strided x, reversed y
+    (9001, 301, 9003, 1, 1, 1.0, 2.0, 1, np.float32),     # unaligned A: the scalar path
+    (4100, 260, 4100, -1, 1, 0.5, 0.25, 0, np.float64),   # FP64: 16-row blocks
+])
+def test_gemv_n_row_blocks(handle, torch_cuda, m, n, lda, incx, incy, alpha, beta, offset, dt):
+    """'n' on tall matrices with a partial last row tile, padded lda, strided
+    and reversed vectors, unaligned data and FP64; every output entry against
+    the FP64 oracle."""
+    _gemv_check(handle, torch_cuda, "n", m, n, alpha, beta, lda, incx, incy, seed=m + n, offset=offset, dtype=dt)
+
+
 @pytest.mark.parametrize("m,n,lda,incx,incy,alpha,beta,poison", [
     (8224, 600, 8224, 1, 1, 1.5, 0.5, False),    # a partial last group of tiles, beta reads y
     (8192, 1000, 8200, 2, -1, -1.0, 0.0, True),  # padded lda, strided x, reversed y, y never read
