@@ -142,7 +142,7 @@ def pmc_probe(torch, pb, h, which="fp16x3"):
                 fn()
                 torch.cuda.synchronize()
                 return 1.0
-        bench_grouped(torch, pb, h, Once())
+        bench_grouped(torch, pb, h, Once(), tune=False)
         log("pmc probe done")
         return
     L = pb.load()
@@ -451,7 +451,7 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def bench_grouped(torch, pb, h, timer):
+def bench_grouped(torch, pb, h, timer, tune=True):
     """512 problems, M, N, K log-uniform in [32, 1024] rounded to multiples of
     8, BF16 inputs, FP32 accumulate/output, alpha 1, beta 0; one persistent
     launch per call, also captured in a graph and replayed."""
@@ -487,6 +487,18 @@ def bench_grouped(torch, pb, h, timer):
     pb.check(L.pb_gemm_grouped_plan_create(h.handle, int(pb.gemm_precision_t.bf16), probs, 512, ctypes.byref(plan)))
     t = timer(lambda: pb.check(L.pb_gemm_grouped_plan_run(h.handle, plan)), reps=20, warm=3)
     out = {"TFLOP/s": flops / t / 1e9, "us": 1e3 * t, "total_flop": flops}
+    if tune:
+        # The schedule picked by the autotuner (every candidate timed on this
+        # problem list); the graph below replays the tuned plan.
+        from portblas_amd import tuning
+
+        best, table = tuning.GroupedAutotuner(h, pb.gemm_precision_t.bf16).tune(probs, 512)
+        pb.check(L.pb_gemm_grouped_plan_destroy(plan))
+        plan = tuning.GroupedAutotuner(h, pb.gemm_precision_t.bf16).plan(probs, 512, best)
+        tt = timer(lambda: pb.check(L.pb_gemm_grouped_plan_run(h.handle, plan)), reps=20, warm=3)
+        out.update({"tuned_us": 1e3 * tt, "tuned_TFLOP/s": flops / tt / 1e9,
+                    "tuned_schedule": dict(zip(("ring", "rank_weights", "spread_div", "per_cu"), best)),
+                    "candidates_timed": len(table)})
     # graph capture on a side stream, replayed
     s = torch.cuda.Stream()
     hs = pb.SB_Handle(s)

@@ -245,6 +245,25 @@ pb_status_t pb_gemm_grouped_plan_destroy(pb_grouped_plan_t plan);
 /* One-shot convenience: create, run, destroy (synchronises before freeing). */
 pb_status_t pb_gemm_grouped(pb_handle_t h, pb_gemm_precision_t precision,
                             const pb_gemm_problem_t* problems, int64_t count);
+/* The schedule choices of a BF16 grouped plan, for the autotuner
+ * (portblas_amd.tuning.GroupedAutotuner times them on the caller's problems).
+ * No reference counterpart.  Zero in every field means the built-in choice,
+ * which pb_gemm_grouped_plan_create uses:
+ *   ring:         1 = the ring kernel for the problems it takes (dimensions
+ *                 multiples of 8, aligned), 2 = the lean kernel for all;
+ *   rank_weights: 1 = weigh the workgroups sharing a CU by their measured
+ *                 speeds, 2 = balance them as equals;
+ *   spread_div:   problems worth more than 1/spread_div of the launch are
+ *                 spread over all XCDs (built-in 16; 1 = never spread);
+ *   per_cu:       workgroups per CU of the grid (built-in: as many as are
+ *                 resident; larger values are clamped to that). */
+typedef struct pb_grouped_options_t {
+  int ring, rank_weights, spread_div, per_cu;
+} pb_grouped_options_t;
+pb_status_t pb_gemm_grouped_plan_create_ex(pb_handle_t h, pb_gemm_precision_t precision,
+                                           const pb_gemm_problem_t* problems, int64_t count,
+                                           const pb_grouped_options_t* options,
+                                           pb_grouped_plan_t* plan);
 
 /* ------------------------------------------------------------------------ */
 /* GEMV: y <- alpha * op(A) * x + beta * y                                   */

@@ -53,6 +53,7 @@ SIGNATURES: dict[str, list] = {
     "pb_pack_fp16x3": [_p, _i32, _ch, _i64, _i64, _p, _i64, _p, _p, _i64, _p],
     "pb_gemm_fp16x3_packed": [_p, _i64, _i64, _i64, _f32, _p, _p, _p, _i64, _p, _p, _p, _i64, _f32, _p, _i64],
     "pb_gemm_grouped_plan_create": [_p, _i32, _p, _i64, ctypes.POINTER(_p)],
+    "pb_gemm_grouped_plan_create_ex": [_p, _i32, _p, _i64, _p, ctypes.POINTER(_p)],
     "pb_gemm_grouped_plan_run": [_p, _p],
     "pb_gemm_grouped_plan_destroy": [_p],
     "pb_gemm_grouped": [_p, _i32, _p, _i64],

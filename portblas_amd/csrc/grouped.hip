@@ -605,8 +605,9 @@ void launch_lean(pb_handle_t h, const Launch& L) {
 // The tiles of one BF16 launch, XCD by XCD.  Workgroup w runs on XCD w % 8,
 // and each of the 8 XCDs has its own L2, so every problem goes whole to one
 // XCD (longest first, to the least-loaded XCD) and its operand panels are
-// fetched into one L2 only.  A problem worth more than a sixteenth of the
-// launch is spread over all XCDs, tile by tile, instead.  Cost of a tile: K
+// fetched into one L2 only.  A problem worth more than 1/spread_div of the
+// launch (built-in: a sixteenth) is spread over all XCDs, tile by tile,
+// instead.  Cost of a tile: K
 // chunks + 1 for the epilogue.  Each XCD's tiles come longest first, a
 // problem's tiles together (the XCD's workgroups work through one problem's
 // panels at a time).
@@ -617,7 +618,8 @@ struct TileItem {
   int4 t;  // problem, tile row, tile column
 };
 
-std::vector<std::vector<TileItem>> xcd_tiles(const std::vector<DevProblem>& v, int xcds, int kchunk) {
+std::vector<std::vector<TileItem>> xcd_tiles(const std::vector<DevProblem>& v, int xcds, int kchunk,
+                                             int spread_div) {
   struct Item { int64_t cost; int prob; };
   std::vector<Item> probs_by_cost;
   int64_t total = 0;
@@ -634,7 +636,7 @@ std::vector<std::vector<TileItem>> xcd_tiles(const std::vector<DevProblem>& v, i
   for (const Item& it : probs_by_cost) {
     const DevProblem& d = v[it.prob];
     const int64_t c = it.cost * int64_t(d.tiles_m) * d.tiles_n;
-    if (xcds > 1 && c * 16 > total) {
+    if (xcds > 1 && spread_div > 1 && c * spread_div > total) {
       for (int32_t tm = 0; tm < d.tiles_m; ++tm)
         for (int32_t tn = 0; tn < d.tiles_n; ++tn) {
           const int x = int(std::min_element(xload.begin(), xload.end()) - xload.begin());
@@ -668,11 +670,11 @@ constexpr double kRankSpeed[3] = {1.0, 0.78, 0.61};
 // each tile's .w holds the K chunks (of kchunk) of that tile and of every
 // later tile of its list (the ring kernel's chunk count), or false if a list
 // has 2^31 chunks or more.  `per_cu`: workgroups per CU in the grid, `cus`:
-// the device's CUs (rank speeds apply when grid == 3 * cus).
+// the device's CUs (rank speeds apply when grid == 3 * cus; per_cu 0: never).
 bool balance_tiles(const std::vector<DevProblem>& v, int grid, int kchunk, int per_cu, int cus,
-                   std::vector<int4>& list, std::vector<int>& starts) {
+                   int spread_div, std::vector<int4>& list, std::vector<int>& starts) {
   const int xcds = grid >= kXcds ? kXcds : 1;
-  const auto by_xcd = xcd_tiles(v, xcds, kchunk);
+  const auto by_xcd = xcd_tiles(v, xcds, kchunk, spread_div);
   const bool ranked = per_cu == 3 && grid == 3 * cus;
   std::vector<double> speed(grid, 1.0);
   if (ranked)
@@ -725,9 +727,23 @@ extern "C" {
 pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
                                         const pb_gemm_problem_t* problems, int64_t count,
                                         pb_grouped_plan_t* out) {
+  return pb_gemm_grouped_plan_create_ex(h, prec, problems, count, nullptr, out);
+}
+
+pb_status_t pb_gemm_grouped_plan_create_ex(pb_handle_t h, pb_gemm_precision_t prec,
+                                           const pb_gemm_problem_t* problems, int64_t count,
+                                           const pb_grouped_options_t* options, pb_grouped_plan_t* out) {
   pb::clear_error();
   PB_TRY(pb::enter(h));
   PB_REQUIRE(out != nullptr, "null plan pointer");
+  const pb_grouped_options_t opt = options ? *options : pb_grouped_options_t{0, 0, 0, 0};
+  PB_REQUIRE(opt.ring >= 0 && opt.ring <= 2 && opt.rank_weights >= 0 && opt.rank_weights <= 2 &&
+                 opt.spread_div >= 0 && opt.per_cu >= 0,
+             "bad grouped options (ring %d, rank_weights %d, spread_div %d, per_cu %d)", opt.ring,
+             opt.rank_weights, opt.spread_div, opt.per_cu);
+  const bool use_ring = opt.ring != 2;
+  const bool rank_weights = opt.rank_weights != 2;
+  const int spread_div = opt.spread_div > 0 ? opt.spread_div : 16;
   *out = nullptr;
   PB_REQUIRE(prec == PB_GEMM_BF16 || prec == PB_GEMM_FP32_MFMA,
              "grouped GEMM supports PB_GEMM_BF16 and PB_GEMM_FP32_MFMA inputs");
@@ -771,7 +787,7 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     d.tiles_n = int32_t(pb::ceil_div(q.n, bn));
     d.vec = aligned16(q.a) && aligned16(q.b) && q.lda % vw == 0 && q.ldb % vw == 0;
     d.c_vec = aligned16(q.c) && q.ldc % 4 == 0;
-    const bool ring = bf16 && d.vec && q.m % 8 == 0 && q.n % 8 == 0 && q.k % 8 == 0 &&
+    const bool ring = use_ring && bf16 && d.vec && q.m % 8 == 0 && q.n % 8 == 0 && q.k % 8 == 0 &&
                       q.lda < kRMaxLd && q.ldb < kRMaxLd && q.k <= 0x7fffffffLL - kRK;
     by_layout[2 * (ta == 't') + (tb == 'n') + (ring ? 4 : 0)].push_back(d);
   }
@@ -833,11 +849,13 @@ pb_status_t pb_gemm_grouped_plan_create(pb_handle_t h, pb_gemm_precision_t prec,
     PB_TRY(upload(v.data(), v.size() * sizeof(DevProblem), &dev));
     L.dev = static_cast<DevProblem*>(dev);
     if (bf16) {
-      const int per_cu = int(lean_per_cu(lay));
+      const int resident = int(lean_per_cu(lay));
+      const int per_cu = opt.per_cu > 0 ? std::min(opt.per_cu, resident) : resident;
       L.grid = int(std::max<int64_t>(1, std::min<int64_t>(t, int64_t(per_cu) * h->num_cus)));
       std::vector<int4> list;
       std::vector<int> starts;
-      if (!balance_tiles(v, L.grid, L.ring ? kRK : kLK, L.ring ? per_cu : 0, h->num_cus, list, starts)) {
+      if (!balance_tiles(v, L.grid, L.ring ? kRK : kLK, L.ring && rank_weights ? per_cu : 0, h->num_cus,
+                         spread_div, list, starts)) {
         destroy();
         return pb::set_error(PB_STATUS_INVALID_VALUE, "grouped BF16 GEMM: 2^31 K chunks or more on one workgroup");
       }

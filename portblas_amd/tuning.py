@@ -427,3 +427,90 @@ def evaluate(chosen: np.ndarray, T: np.ndarray) -> dict:
         "p90_slowdown": float(np.quantile(ratio, 0.9)),
         "total_time_ratio": float(T[rows, chosen].sum() / T.min(axis=1).sum()),
     }
+
+
+# ---------------------------------------------------------------------------
+# Grouped BF16 GEMM: the schedule choices of a plan (pb_grouped_options_t),
+# timed on the caller's own problems; the fastest is kept.  The reference has
+# no grouped GEMM; its tuner's method (time every candidate, keep the
+# fastest: tools/auto_tuner/include/gemm_tuner.hpp:99-180) is applied to the
+# plan's schedule.
+# ---------------------------------------------------------------------------
+class GroupedOptions(ctypes.Structure):
+    """pb_grouped_options_t: 0 in a field = the built-in choice."""
+    _fields_ = [("ring", ctypes.c_int), ("rank_weights", ctypes.c_int), ("spread_div", ctypes.c_int),
+                ("per_cu", ctypes.c_int)]
+
+    def as_tuple(self):
+        return (self.ring, self.rank_weights, self.spread_div, self.per_cu)
+
+
+class GroupedProblem(ctypes.Structure):
+    """pb_gemm_problem_t."""
+    _fields_ = [("transa", ctypes.c_char), ("transb", ctypes.c_char), ("m", ctypes.c_int64),
+                ("n", ctypes.c_int64), ("k", ctypes.c_int64), ("alpha", ctypes.c_float),
+                ("beta", ctypes.c_float), ("a", ctypes.c_void_p), ("lda", ctypes.c_int64),
+                ("b", ctypes.c_void_p), ("ldb", ctypes.c_int64), ("c", ctypes.c_void_p),
+                ("ldc", ctypes.c_int64)]
+
+
+def grouped_candidates():
+    """(ring, rank_weights, spread_div, per_cu): the built-in schedule first,
+    then the ring or lean kernel, CU-position weights on or off, the XCD
+    spread threshold (1/16, 1/4, never) and 3 or 2 workgroups per CU."""
+    out = [(0, 0, 0, 0)]
+    for ring in (1, 2):
+        for rank in (1, 2):
+            for spread in (16, 4, 1):
+                for per_cu in (3, 2):
+                    if ring == 2 and rank == 2:
+                        continue  # the lean kernel never weighs ranks
+                    out.append((ring, rank, spread, per_cu))
+    return out
+
+
+class GroupedAutotuner:
+    """Times grouped plans built with each candidate schedule on one problem
+    list (HIP events around pb_gemm_grouped_plan_run on the handle's stream)
+    and keeps the fastest.  The plans run repeatedly, so every problem must
+    have beta == 0 (C is then overwritten with the same result each time)."""
+
+    def __init__(self, handle, precision: int):
+        self.h = handle
+        self.L = _lib.load()
+        self.precision = int(precision)
+
+    def plan(self, problems, count: int, opts) -> ctypes.c_void_p:
+        o = GroupedOptions(*opts)
+        plan = ctypes.c_void_p()
+        check(self.L.pb_gemm_grouped_plan_create_ex(self.h.handle, self.precision, problems, count,
+                                                    ctypes.byref(o), ctypes.byref(plan)))
+        return plan
+
+    def time(self, problems, count: int, opts, warm: int = 2, reps: int = 5) -> float:
+        import torch
+
+        plan = self.plan(problems, count, opts)
+        try:
+            for _ in range(warm):
+                check(self.L.pb_gemm_grouped_plan_run(self.h.handle, plan))
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                check(self.L.pb_gemm_grouped_plan_run(self.h.handle, plan))
+            e1.record()
+            e1.synchronize()
+            return e0.elapsed_time(e1) * 1e3 / reps  # us
+        finally:
+            check(self.L.pb_gemm_grouped_plan_destroy(plan))
+
+    def tune(self, problems, count: int, candidates=None, warm: int = 2, reps: int = 5):
+        """Returns (best options tuple, {options: us})."""
+        if any(problems[i].beta != 0.0 for i in range(count)):
+            raise ValueError("grouped autotuning runs each plan repeatedly: every problem needs beta == 0")
+        table = {}
+        for opts in candidates or grouped_candidates():
+            table[tuple(opts)] = self.time(problems, count, opts, warm, reps)
+        best = min(table, key=table.get)
+        return best, table

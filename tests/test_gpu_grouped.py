@@ -219,3 +219,46 @@ def test_graph_of_blas_calls(handle, torch_cuda):
             assert torch.equal(C, C_e) and torch.equal(y, y_e) and torch.equal(rs, rs_e), rep
     finally:
         L.pb_graph_destroy(graph)
+
+
+def test_grouped_schedule_options_and_autotuner(handle, torch_cuda):
+    """Every schedule the grouped autotuner times (ring or lean kernel, CU-
+    position weights, XCD spread threshold, workgroups per CU) gives C within
+    the FP32 bound of the FP64 product, identical across schedules (each tile
+    accumulates its K chunks in the same order whatever workgroup runs it);
+    the tuner returns one of its candidates, refuses beta != 0 problems, and
+    the library refuses bad option values."""
+    import portblas_amd as pb
+    from portblas_amd import tuning
+
+    torch = torch_cuda
+    rng = np.random.default_rng(11)
+    dims = [tuple(int(v) for v in np.maximum(8, np.round(np.exp(rng.uniform(np.log(24), np.log(700), 3)) / 8) * 8))
+            for _ in range(40)]
+    dims += [(37, 45, 53), (100, 3, 77)]  # not multiples of 8: the lean kernel
+    probs, bufs = _make_problems(torch, dims, torch.bfloat16, seed=21, beta_zero=True)
+    tuner = tuning.GroupedAutotuner(handle, pb.gemm_precision_t.bf16)
+    L = pb.load()
+    outs = []
+    for opts in tuning.grouped_candidates():
+        plan = tuner.plan(probs, len(dims), opts)
+        pb.check(L.pb_gemm_grouped_plan_run(handle.handle, plan))
+        torch.cuda.synchronize()
+        pb.check(L.pb_gemm_grouped_plan_destroy(plan))
+        _check(bufs)
+        outs.append([b[11].clone() for b in bufs])
+    for o in outs[1:]:
+        for x, y in zip(outs[0], o):
+            assert torch.equal(x, y)
+    best, table = tuner.tune(probs, len(dims), reps=2, warm=1)
+    assert best in table and set(table) == set(tuning.grouped_candidates())
+    assert all(t > 0 for t in table.values())
+    probs[0].beta = 0.5
+    with pytest.raises(ValueError):
+        tuner.tune(probs, len(dims))
+    probs[0].beta = 0.0
+    bad = tuning.GroupedOptions(3, 0, 0, 0)
+    plan = ctypes.c_void_p()
+    st = L.pb_gemm_grouped_plan_create_ex(handle.handle, int(pb.gemm_precision_t.bf16), probs, len(dims),
+                                          ctypes.byref(bad), ctypes.byref(plan))
+    assert st == 1 and not plan.value  # PB_STATUS_INVALID_VALUE
