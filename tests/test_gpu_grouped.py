@@ -262,3 +262,38 @@ def test_grouped_schedule_options_and_autotuner(handle, torch_cuda):
     st = L.pb_gemm_grouped_plan_create_ex(handle.handle, int(pb.gemm_precision_t.bf16), probs, len(dims),
                                           ctypes.byref(bad), ctypes.byref(plan))
     assert st == 1 and not plan.value  # PB_STATUS_INVALID_VALUE
+
+
+def test_grouped_ring_edges(handle, torch_cuda):
+    """K = 0 on both kernels (C = beta C), a plan too small to fill the XCD
+    groups (one and three problems), and a ring problem whose leading
+    dimension is just under the ring kernel's 2^24 limit, so that a thread's
+    byte offset in its panel comes within 2 % of 2^31."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    for dims in ([(64, 64, 0), (8, 16, 0), (37, 21, 0), (128, 64, 64)], [(72, 40, 24)],
+                 [(8, 8, 8), (16, 24, 40), (64, 8, 0)]):
+        probs, bufs = _make_problems(torch, dims, torch.bfloat16, seed=31 + len(dims))
+        pb.check(L.pb_gemm_grouped(handle.handle, int(pb.gemm_precision_t.bf16), probs, len(dims)))
+        torch.cuda.synchronize()
+        _check(bufs)
+    # A = op(A)^T stored K-contiguous ('t'): row r of the 64-row tile starts
+    # r * lda elements in; lda = 2^24 - 8 puts row 63 at 2.11e9 bytes.
+    g = torch.Generator(device="cuda").manual_seed(5)
+    m, n, k, lda = 64, 64, 16, (1 << 24) - 8
+    a = torch.zeros((m - 1) * lda + k, dtype=torch.bfloat16, device="cuda")
+    rows = torch.arange(m, device="cuda") * lda
+    vals = (torch.rand(m, k, generator=g, device="cuda") * 2 - 1).to(torch.bfloat16)
+    a.view(-1)[(rows[:, None] + torch.arange(k, device="cuda")[None, :]).reshape(-1)] = vals.reshape(-1)
+    b = (torch.rand(k * n, generator=g, device="cuda") * 2 - 1).to(torch.bfloat16)
+    c = torch.zeros(m * n, device="cuda")
+    p = (Prob * 1)(Prob(b"t", b"n", m, n, k, 1.0, 0.0, a.data_ptr(), lda, b.data_ptr(), k, c.data_ptr(), m))
+    pb.check(L.pb_gemm_grouped(handle.handle, int(pb.gemm_precision_t.bf16), p, 1))
+    torch.cuda.synchronize()
+    ref = vals.double() @ b.double().view(n, k).t()
+    tol = 1.01 * (k + 2) * U * (vals.double().abs() @ b.double().view(n, k).t().abs()) + 1e-30
+    got = c.view(n, m).t().double()
+    assert ((got - ref).abs() <= tol).all(), ((got - ref).abs() / tol).max().item()
+    del a
