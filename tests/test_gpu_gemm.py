@@ -285,13 +285,44 @@ def test_strided_batched_4096_x_64(handle, torch_cuda):
         assert np.all(np.abs(Cn[sl] - ref) <= bound(s, 1.0, ap, 0.0, 0.0, U32))
 
 
+@pytest.mark.parametrize("ta,tb", [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")])
+@pytest.mark.parametrize("batch,pad", [(256, 0), (300, 4), (1031, 0), (4099, 0)])
+def test_small_64_cubed_beta_zero(handle, torch_cuda, ta, tb, batch, pad):
+    """FP32 64^3 batches with beta = 0 run on the persistent kernel that keeps
+    the next problems' operands in flight: one to 17 problems per workgroup,
+    every transpose, tight and padded leading dimensions.  C starts as NaN:
+    beta = 0 must not read it, and every element is written."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    s = 64
+    ld = s + pad
+    sa = sb = ld * s
+    sc = ld * s
+    A = oracle.uniform(31, sa * batch).astype(np.float32)
+    B = oracle.uniform(32, sb * batch).astype(np.float32)
+    dC = torch.full((sc * batch,), float("nan"), device="cuda")
+    pb._gemm_strided_batched(handle, ta, tb, s, s, s, -1.5, to_dev(torch, A), ld, sa, to_dev(torch, B), ld, sb,
+                             0.0, dC, ld, sc, batch)
+    got = dC.cpu().numpy().astype(np.float64).reshape(batch, s, ld)[:, :, :s]
+    assert np.all(np.isfinite(got))
+    zero = np.zeros(sc, np.float32)
+    for b in [0, 1, 255, batch // 2, batch - 2, batch - 1]:
+        ref = oracle.gemm_f64(ta, tb, s, s, s, -1.5, A[b * sa:], ld, B[b * sb:], ld, 0.0, zero, ld)
+        ap = colmajor(oracle.gemm_absprod_f64(ta, tb, s, s, s, A[b * sa:], ld, B[b * sb:], ld), s, s, s)
+        e = np.abs(got[b].T - colmajor(ref, s, s, ld))
+        assert np.all(e <= bound(s, -1.5, ap, 0.0, 0.0, U32)), (b, float(np.max(e)))
+
+
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("ta,tb", [("n", "n"), ("n", "t"), ("t", "n"), ("t", "t")])
-@pytest.mark.parametrize("m,n,k,lda_pad", [(64, 64, 64, 0), (64, 64, 192, 0), (37, 50, 200, 3), (8, 61, 5, 1)])
+@pytest.mark.parametrize("m,n,k,lda_pad", [(64, 64, 64, 0), (64, 64, 192, 0), (64, 64, 128, 4), (37, 50, 200, 3),
+                                             (8, 61, 5, 1)])
 def test_many_small_problems(handle, torch_cuda, dtype, ta, tb, m, n, k, lda_pad):
     """Batches of >= one problem per CU with M, N <= 64 take the
-    one-problem-per-workgroup kernel: full 64 x 64 tiles with K in one or
-    three chunks of 64, ragged M/N, K over several chunks,
+    one-problem-per-workgroup kernels: full 64 x 64 tiles with K in one to
+    three chunks of 64 (tight and padded, 16-byte aligned leading
+    dimensions), ragged M/N, K over several chunks,
     unaligned leading dimensions (scalar staging), alpha/beta; 300 problems,
     checked on a sample against the FP64 oracle."""
     import portblas_amd as pb
