@@ -258,6 +258,23 @@ def main():
     ms_per_step = 1e3 * elapsed / args.steps
     value = flop / (ms_per_step * 1e-3) / 1e12
     log(f"fp16x3 sgemm {N}^3: {ms_per_step:.3f} ms/step, {value:.1f} TFLOP/s")
+    # The FP16x3 pre-pass packs with the previous call's scale exponents and
+    # packs an operand again when its amax gives another one.  The steps above
+    # repeat one input pair (every guess holds); alternating with a pair scaled
+    # by 2^12 and 2^-12 makes every guess miss.
+    A2, B2 = A * 4096.0, B * (1.0 / 4096.0)
+    flip = [False]
+
+    def miss_step():
+        flip[0] = not flip[0]
+        a, b = (A2, B2) if flip[0] else (A, B)
+        pb._gemm_ex(h, pb.gemm_precision_t.fp16x3, "n", "n", N, N, N, 1.0, a, N, b, N, 0.0, C, N)
+
+    t_miss = timer(miss_step, reps=4, warm=1)
+    res["fp16x3_scale_miss_ms"] = t_miss
+    res["fp16x3_scale_miss_tflops"] = flop / t_miss / 1e9
+    log(f"fp16x3 with both scale guesses missing: {t_miss:.3f} ms ({res['fp16x3_scale_miss_tflops']:.1f} TFLOP/s)")
+    del A2, B2
 
     # plain FP32 MFMA and BF16x3 on the same inputs
     t_fp32 = timer(lambda: pb._gemm(h, "n", "n", N, N, N, 1.0, A, N, B, N, 0.0, C, N), reps=3)

@@ -196,6 +196,62 @@ __global__ __launch_bounds__(kAmaxBlock) void f16_scales_kernel(const float* pa,
   if (factor) factor[0] = ldexpf(1.0f, -ea);
 }
 
+// FP16x3 scales from packs that used guessed exponents guess[0..1]: writes
+// scales as f16_scales_kernel does, flags[i] = 1 where the true exponent
+// differs from the guess (that operand is packed again), and the true
+// exponents into guess for the next call.
+// One workgroup of kCheckBlock threads, 8 independent loads in flight per
+// thread and operand (the packs leave one partial per workgroup: 8192 per
+// 8192^2 operand).
+constexpr int kCheckBlock = 1024;
+__global__ __launch_bounds__(kCheckBlock) void f16_check_kernel(const float* pa, int na, const float* pb, int nb,
+                                                                int* guess, float* scales, int* flags) {
+  __shared__ float red[2][kCheckBlock / 64];
+  constexpr int U = 8;
+  float m[2][U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) m[0][u] = m[1][u] = 0.f;
+  for (int64_t i0 = threadIdx.x; i0 < (na > nb ? na : nb); i0 += kCheckBlock * U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      const int64_t i = i0 + u * kCheckBlock;
+      if (i < na) m[0][u] = fmaxf(m[0][u], pa[i]);
+      if (i < nb) m[1][u] = fmaxf(m[1][u], pb[i]);
+    }
+  }
+  float a = m[0][0], b = m[1][0];
+#pragma unroll
+  for (int u = 1; u < U; ++u) {
+    a = fmaxf(a, m[0][u]);
+    b = fmaxf(b, m[1][u]);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    a = fmaxf(a, __shfl_xor(a, off, 64));
+    b = fmaxf(b, __shfl_xor(b, off, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    red[0][threadIdx.x >> 6] = a;
+    red[1][threadIdx.x >> 6] = b;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  float am[2] = {red[0][0], red[1][0]};
+#pragma unroll
+  for (int w = 1; w < kCheckBlock / 64; ++w) {
+    am[0] = fmaxf(am[0], red[0][w]);
+    am[1] = fmaxf(am[1], red[1][w]);
+  }
+  const int ea = f16_scale_exp(am[0]), eb = f16_scale_exp(am[1]);
+  scales[0] = ldexpf(1.0f, ea);
+  scales[1] = ldexpf(1.0f, eb);
+  scales[2] = ldexpf(1.0f, -(ea + eb));
+  flags[0] = ea != guess[0];
+  flags[1] = eb != guess[1];
+  guess[0] = ea;
+  guess[1] = eb;
+}
+
 enum PackMode { kPackBf16 = 0, kPackBf16Split = 1, kPackFp8 = 2, kPackF16Split = 3 };
 
 // Modes that multiply by a per-tensor scale (scales[scale_idx]) first.
@@ -274,88 +330,169 @@ __device__ __forceinline__ void pack_convert(float v, uint16_t& hi, uint16_t& lo
   }
 }
 
+// Where pack_vec_kernel takes its scale from.
+enum PackScale { kScaleGiven = 0, kScaleGuess = 1, kScaleRepack = 2 };
+
+// 64-wide K tiles that one workgroup of pack_vec_kernel converts in turn.
+constexpr int kPackKTiles = 2;
+
 // Vectorized pre-pass for 16-byte aligned X with 4-element aligned pitches.
-// Same contract as pack_kernel.  A 64 x 64 tile of op(X) is read with
-// 16-byte loads along X's contiguous dimension (KC: along k; else along the
-// rows) into LDS as [row][k]; then thread t converts the 16 consecutive k
-// 16 (t / 64) .. +15 of row t % 64, so 64 neighbouring threads write 64
-// consecutive rows of one K slab (contiguous in the slab-major layout).
-template <int MODE, bool KC>
+// Same contract as pack_kernel, with grid.y covering kPackKTiles K tiles per
+// workgroup.  A 64 x 64 tile of op(X) is read with 16-byte loads along X's
+// contiguous dimension (KC: along k; else along the rows) into LDS as
+// [row][k]; then thread t converts the 16 consecutive k 16 (t / 64) .. +15 of
+// row t % 64, so 64 neighbouring threads write 64 consecutive rows of one K
+// slab (contiguous in the slab-major layout).  The next tile's loads are in
+// flight while the current one is converted and written.
+//
+// SC selects where the FP16x3 scale comes from.  kScaleGiven: scales[scale_idx].
+// kScaleGuess: 2^(*ctl), a guess of the operand's scale exponent; the
+// workgroup also writes the finite amax of its elements to
+// amax_part[linear block index].  kScaleRepack: scales[scale_idx], and only
+// when *ctl is nonzero (the guess was wrong); otherwise every workgroup exits.
+template <int MODE, bool KC, int SC = kScaleGiven>
 __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__ x, int64_t rs,
                                                        int64_t ks, int64_t rows, int64_t K,
                                                        int64_t stride, int64_t kp, int slab,
                                                        void* dst_hi, void* dst_lo,
-                                                       const float* scales, int scale_idx) {
+                                                       const float* scales, int scale_idx,
+                                                       const int* ctl, float* amax_part) {
+  if constexpr (SC == kScaleRepack)
+    if (*ctl == 0) return;
   __shared__ float tile[64][65];  // [row][k]
   const int64_t r0 = int64_t(blockIdx.x) * 64;
-  const int64_t k0 = int64_t(blockIdx.y) * 64;
+  const int64_t kt0 = int64_t(blockIdx.y) * kPackKTiles;
   const int64_t b = blockIdx.z;
   const float* xb = x + b * stride;
+  int64_t nkt = (kp + 63) / 64 - kt0;
+  if (nkt > kPackKTiles) nkt = kPackKTiles;
+  float v[4][4];
+  auto load = [&](int64_t k0) {
+    if (r0 + 64 <= rows && k0 + 64 <= K) {
+      // Interior tile (the same branch for the whole workgroup): four
+      // unconditional 16-byte loads in flight together.  With per-lane
+      // conditions the compiler waits for each load before the next one that
+      // writes the same registers on the other path.
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int idx = threadIdx.x + 256 * i;
-    // KC: row idx / 16, k 4 (idx % 16); else k idx / 16, row 4 (idx % 16).
-    const int rr = KC ? idx / 16 : 4 * (idx % 16);
-    const int kk = KC ? 4 * (idx % 16) : idx / 16;
-    const int64_t r = r0 + rr, k = k0 + kk;
-    float v[4] = {0.f, 0.f, 0.f, 0.f};
-    if constexpr (KC) {
-      if (r < rows) {
-        if (k + 3 < K) {
-          const pbdev::f32x4_t t =
-              pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r * rs + k));
-          v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) v[e] = (k + e < K) ? xb[r * rs + k + e] : 0.f;
-        }
+      for (int i = 0; i < 4; ++i) {
+        const int idx = threadIdx.x + 256 * i;
+        const int rr = KC ? idx / 16 : 4 * (idx % 16);
+        const int kk = KC ? 4 * (idx % 16) : idx / 16;
+        const float* src = KC ? xb + (r0 + rr) * rs + k0 + kk : xb + r0 + rr + (k0 + kk) * ks;
+        const pbdev::f32x4_t t = pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(src));
+        v[i][0] = t.x; v[i][1] = t.y; v[i][2] = t.z; v[i][3] = t.w;
       }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) tile[rr][kk + e] = v[e];
-    } else {
-      if (k < K) {
-        if (r + 3 < rows) {
-          const pbdev::f32x4_t t =
-              pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r + k * ks));
-          v[0] = t.x; v[1] = t.y; v[2] = t.z; v[3] = t.w;
-        } else {
-#pragma unroll
-          for (int e = 0; e < 4; ++e) v[e] = (r + e < rows) ? xb[r + e + k * ks] : 0.f;
-        }
-      }
-#pragma unroll
-      for (int e = 0; e < 4; ++e) tile[rr + e][kk] = v[e];
+      return;
     }
-  }
-  __syncthreads();
-  const float s = scaled_mode(MODE) ? scales[scale_idx] : 1.0f;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      // KC: row idx / 16, k 4 (idx % 16); else k idx / 16, row 4 (idx % 16).
+      const int rr = KC ? idx / 16 : 4 * (idx % 16);
+      const int kk = KC ? 4 * (idx % 16) : idx / 16;
+      const int64_t r = r0 + rr, k = k0 + kk;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) v[i][e] = 0.f;
+      if constexpr (KC) {
+        if (r < rows) {
+          if (k + 3 < K) {
+            const pbdev::f32x4_t t =
+                pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r * rs + k));
+            v[i][0] = t.x; v[i][1] = t.y; v[i][2] = t.z; v[i][3] = t.w;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[i][e] = (k + e < K) ? xb[r * rs + k + e] : 0.f;
+          }
+        }
+      } else {
+        if (k < K) {
+          if (r + 3 < rows) {
+            const pbdev::f32x4_t t =
+                pbdev::ld_stream(reinterpret_cast<const pbdev::f32x4_t*>(xb + r + k * ks));
+            v[i][0] = t.x; v[i][1] = t.y; v[i][2] = t.z; v[i][3] = t.w;
+          } else {
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[i][e] = (r + e < rows) ? xb[r + e + k * ks] : 0.f;
+          }
+        }
+      }
+    }
+  };
+  auto to_lds = [&]() {
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int idx = threadIdx.x + 256 * i;
+      const int rr = KC ? idx / 16 : 4 * (idx % 16);
+      const int kk = KC ? 4 * (idx % 16) : idx / 16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        if constexpr (KC) tile[rr][kk + e] = v[i][e];
+        else tile[rr + e][kk] = v[i][e];
+      }
+    }
+  };
+  const float s = SC == kScaleGuess ? ldexpf(1.0f, *ctl) : scaled_mode(MODE) ? scales[scale_idx] : 1.0f;
+  float best = 0.f;  // finite amax of the converted elements (kScaleGuess)
   const int rr = threadIdx.x % 64, kq = 16 * (threadIdx.x / 64);
-  const int64_t r = r0 + rr, k = k0 + kq;
-  if (r >= rows || k >= kp) return;
-  // 16 consecutive k of one row are contiguous in every packed layout (kp
-  // and the slab width are multiples of 16).
-  const int64_t o = packed_index(b, r, k, rows, kp, slab);
-  uint16_t h[16], l[16];
-  uint8_t q[16];
+  auto convert = [&](int64_t k0) {
+    const int64_t r = r0 + rr, k = k0 + kq;
+    if (r >= rows || k >= kp) return;
+    // 16 consecutive k of one row are contiguous in every packed layout (kp
+    // and the slab width are multiples of 16).
+    const int64_t o = packed_index(b, r, k, rows, kp, slab);
+    uint16_t h[16], l[16];
+    uint8_t q[16];
 #pragma unroll
-  for (int e = 0; e < 16; ++e) pack_convert<MODE>(tile[rr][kq + e] * s, h[e], l[e], q[e]);
-  if constexpr (MODE == kPackFp8) {
-    uint4 w;
-    __builtin_memcpy(&w, q, 16);
-    *reinterpret_cast<uint4*>(static_cast<uint8_t*>(dst_hi) + o) = w;
-  } else {
-    uint4 w[2];
-    __builtin_memcpy(w, h, 32);
-    uint4* dh = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_hi) + o);
-    dh[0] = w[0];
-    dh[1] = w[1];
-    if constexpr (MODE == kPackBf16Split || MODE == kPackF16Split) {
-      __builtin_memcpy(w, l, 32);
-      uint4* dl = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_lo) + o);
-      dl[0] = w[0];
-      dl[1] = w[1];
+    for (int e = 0; e < 16; ++e) {
+      const float t = tile[rr][kq + e];
+      // Every element of op(X) is converted by exactly one thread.
+      if constexpr (SC == kScaleGuess) best = fmaxf(best, amag<true>(t));
+      pack_convert<MODE>(t * s, h[e], l[e], q[e]);
     }
+    if constexpr (MODE == kPackFp8) {
+      uint4 w;
+      __builtin_memcpy(&w, q, 16);
+      *reinterpret_cast<uint4*>(static_cast<uint8_t*>(dst_hi) + o) = w;
+    } else {
+      uint4 w[2];
+      __builtin_memcpy(w, h, 32);
+      uint4* dh = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_hi) + o);
+      dh[0] = w[0];
+      dh[1] = w[1];
+      if constexpr (MODE == kPackBf16Split || MODE == kPackF16Split) {
+        __builtin_memcpy(w, l, 32);
+        uint4* dl = reinterpret_cast<uint4*>(static_cast<uint16_t*>(dst_lo) + o);
+        dl[0] = w[0];
+        dl[1] = w[1];
+      }
+    }
+  };
+  // The barriers are written out: __syncthreads() would also wait for the
+  // previous tile's stores and the next tile's loads.
+  load(kt0 * 64);
+  for (int64_t t = 0; t < nkt; ++t) {
+    if (t > 0) asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");  // tile consumed
+    to_lds();
+    asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+    if (t + 1 < nkt) load((kt0 + t + 1) * 64);
+    convert((kt0 + t) * 64);
   }
+  if constexpr (SC == kScaleGuess) {
+    __shared__ float red[4];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) best = fmaxf(best, __shfl_xor(best, off, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = best;
+    __syncthreads();
+    if (threadIdx.x == 0)
+      amax_part[blockIdx.x + int64_t(gridDim.x) * (blockIdx.y + int64_t(gridDim.y) * blockIdx.z)] =
+          fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+  }
+}
+
+// Grid of pack_vec_kernel for op(X) with `rows` rows and packed width kp.
+inline dim3 pack_vec_grid(int64_t rows, int64_t kp, int64_t batch) {
+  return dim3{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(pb::ceil_div(kp, 64), kPackKTiles)),
+              unsigned(batch)};
 }
 
 // ---------------------------------------------------------------------------
@@ -556,8 +693,29 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
     PB_REQUIRE(id >= 0 && id < kNumConfigs, "config id %d out of range", id);
     part_bytes = config_splitk_bytes(id, M, N, K, batch);
   }
+  // Operand strides of op(A) (rows M) and op(B) (rows N), as for the engine.
+  const int64_t a_rs = g.ta == 'n' ? 1 : g.lda, a_ks = g.ta == 'n' ? g.lda : 1;
+  const int64_t b_rs = g.tb == 'n' ? g.ldb : 1, b_ks = g.tb == 'n' ? 1 : g.ldb;
+  PB_REQUIRE(batch <= 65535, "batch too large for the pre-pass");
+  // 16-byte loads need an aligned base and 4-element pitches; the packed
+  // pitch kp is a multiple of 8.
+  auto vec_ok = [&](const float* x, int64_t rs, int64_t ks, int64_t stride) {
+    const int64_t pitch = ks == 1 ? rs : ks;
+    return aligned16(x) && pitch % 4 == 0 && (batch == 1 || stride % 4 == 0) && (rs == 1 || ks == 1);
+  };
+  // FP16x3 with both operands on the vectorized pre-pass skips the amax
+  // passes: each operand is packed with the scale exponent of the handle's
+  // previous FP16x3 call while the pack takes its amax; a check kernel then
+  // sets the true scales and flags an operand whose exponent changed, and
+  // that operand alone is packed again (the other repack launch exits at
+  // once).  The packed planes and scales are those of the two-pass order.
+  const bool guess = f16s && vec_ok(g.a, a_rs, a_ks, g.sa) && vec_ok(g.b, b_rs, b_ks, g.sb);
+  const dim3 ga = pack_vec_grid(M, kp, batch), gb = pack_vec_grid(N, kp, batch);
+  const int64_t gna = int64_t(ga.x) * ga.y * ga.z, gnb = int64_t(gb.x) * gb.y * gb.z;
+  PB_REQUIRE(!guess || (gna <= INT32_MAX && gnb <= INT32_MAX), "operand too large for the pre-pass");
+  const size_t guess_bytes = guess ? pb::align_up(4 * size_t(gna), 256) + pb::align_up(4 * size_t(gnb), 256) + 256 : 0;
   void* ws = nullptr;
-  PB_TRY(pb::workspace(h, nparts * (abytes + bbytes) + part_bytes + 1024, &ws));
+  PB_TRY(pb::workspace(h, nparts * (abytes + bbytes) + part_bytes + guess_bytes + 1024, &ws));
   Scratch sc{static_cast<uint8_t*>(ws)};
   void* a_hi = sc.take<uint8_t>(abytes);
   void* a_lo = nparts == 2 ? sc.take<uint8_t>(abytes) : nullptr;
@@ -567,71 +725,92 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   float* amax_part = sc.take<float>(2 * kAmaxMaxParts);
   float* partial = part_bytes ? sc.take<float>(part_bytes / sizeof(float)) : nullptr;
 
-  // Operand strides of op(A) (rows M) and op(B) (rows N), as for the engine.
-  const int64_t a_rs = g.ta == 'n' ? 1 : g.lda, a_ks = g.ta == 'n' ? g.lda : 1;
-  const int64_t b_rs = g.tb == 'n' ? g.ldb : 1, b_ks = g.tb == 'n' ? 1 : g.ldb;
-  if (fp8 || f16s) {
-    // amax of each operand over all batches (one partial per workgroup),
-    // then the scales (device side).  Dense operands stream as one block;
-    // others go line by line along their contiguous dimension.  The FP16
-    // split's amax counts finite elements only.
-    auto amax = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, float* part) -> int {
-      return f16s ? launch_amax<true>(h, x, rs, ks, rows, K, stride, batch, part)
-                  : launch_amax<false>(h, x, rs, ks, rows, K, stride, batch, part);
+  if (guess) {
+    float* pa = sc.take<float>(size_t(gna));
+    float* pbp = sc.take<float>(size_t(gnb));
+    int* flags = sc.take<int>(2);
+    const bool kca = a_ks == 1, kcb = b_ks == 1;
+    auto packg = [&](auto sc_mode, bool kc, dim3 grid, const float* x, int64_t rs, int64_t ks, int64_t rows,
+                     int64_t stride, void* hi, void* lo, int sidx, const int* ctl, float* part) {
+      constexpr int SCM = decltype(sc_mode)::value;
+      if (kc)
+        hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true, SCM>), grid, dim3(256), 0, h->stream, x, rs, ks,
+                           rows, K, stride, kp, slab, hi, lo, static_cast<const float*>(scales), sidx, ctl, part);
+      else
+        hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false, SCM>), grid, dim3(256), 0, h->stream, x, rs,
+                           ks, rows, K, stride, kp, slab, hi, lo, static_cast<const float*>(scales), sidx, ctl,
+                           part);
     };
-    const int na = amax(g.a, a_rs, a_ks, M, g.sa, amax_part);
-    const int nb = amax(g.b, b_rs, b_ks, N, g.sb, amax_part + kAmaxMaxParts);
-    if (f16s)
-      hipLaunchKernelGGL(f16_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
-                         amax_part + kAmaxMaxParts, nb, scales, static_cast<float*>(nullptr));
-    else
-      hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
-                         amax_part + kAmaxMaxParts, nb, scales);
+    using Guess = std::integral_constant<int, kScaleGuess>;
+    using Repack = std::integral_constant<int, kScaleRepack>;
+    packg(Guess{}, kca, ga, g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0, h->f16_guess, pa);
+    packg(Guess{}, kcb, gb, g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1, h->f16_guess + 1, pbp);
+    hipLaunchKernelGGL(f16_check_kernel, dim3(1), dim3(kCheckBlock), 0, h->stream, pa, int(gna), pbp, int(gnb),
+                       h->f16_guess, scales, flags);
+    packg(Repack{}, kca, ga, g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0, flags, static_cast<float*>(nullptr));
+    packg(Repack{}, kcb, gb, g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1, flags + 1, static_cast<float*>(nullptr));
+    PB_CHECK_LAUNCH();
+  } else {
+    if (fp8 || f16s) {
+      // amax of each operand over all batches (one partial per workgroup),
+      // then the scales (device side).  Dense operands stream as one block;
+      // others go line by line along their contiguous dimension.  The FP16
+      // split's amax counts finite elements only.
+      auto amax = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, float* part) -> int {
+        return f16s ? launch_amax<true>(h, x, rs, ks, rows, K, stride, batch, part)
+                    : launch_amax<false>(h, x, rs, ks, rows, K, stride, batch, part);
+      };
+      const int na = amax(g.a, a_rs, a_ks, M, g.sa, amax_part);
+      const int nb = amax(g.b, b_rs, b_ks, N, g.sb, amax_part + kAmaxMaxParts);
+      if (f16s)
+        hipLaunchKernelGGL(f16_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
+                           amax_part + kAmaxMaxParts, nb, scales, static_cast<float*>(nullptr));
+      else
+        hipLaunchKernelGGL(fp8_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, amax_part, na,
+                           amax_part + kAmaxMaxParts, nb, scales);
+      PB_CHECK_LAUNCH();
+    }
+    auto pack = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, void* hi,
+                    void* lo, int sidx) {
+      dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(kp, 64)), unsigned(batch)};
+      const bool kc = ks == 1;
+      const int* none_i = nullptr;
+      float* none_f = nullptr;
+      if (vec_ok(x, rs, ks, stride)) {
+        const dim3 vgrid = pack_vec_grid(rows, kp, batch);
+#define PB_PACKV(MODE)                                                                       \
+    do {                                                                                       \
+      if (kc)                                                                                  \
+        hipLaunchKernelGGL((pack_vec_kernel<MODE, true>), vgrid, dim3(256), 0, h->stream, x, rs, \
+                           ks, rows, K, stride, kp, slab, hi, lo, scales, sidx, none_i, none_f); \
+      else                                                                                     \
+        hipLaunchKernelGGL((pack_vec_kernel<MODE, false>), vgrid, dim3(256), 0, h->stream, x,  \
+                           rs, ks, rows, K, stride, kp, slab, hi, lo, scales, sidx, none_i, none_f); \
+    } while (0)
+        if (f16s) PB_PACKV(kPackF16Split);
+        else if (split) PB_PACKV(kPackBf16Split);
+        else if (prec == PB_GEMM_BF16) PB_PACKV(kPackBf16);
+        else PB_PACKV(kPackFp8);
+#undef PB_PACKV
+        return;
+      }
+      if (f16s)
+        hipLaunchKernelGGL(pack_kernel<kPackF16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
+                           rows, K, stride, kp, slab, hi, lo, scales, sidx);
+      else if (split)
+        hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
+                           rows, K, stride, kp, slab, hi, lo, scales, sidx);
+      else if (prec == PB_GEMM_BF16)
+        hipLaunchKernelGGL(pack_kernel<kPackBf16>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
+                           stride, kp, slab, hi, lo, scales, sidx);
+      else
+        hipLaunchKernelGGL(pack_kernel<kPackFp8>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
+                           stride, kp, slab, hi, lo, scales, sidx);
+    };
+    pack(g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0);
+    pack(g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1);
     PB_CHECK_LAUNCH();
   }
-  PB_REQUIRE(batch <= 65535, "batch too large for the pre-pass");
-  auto pack = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride, void* hi,
-                  void* lo, int sidx) {
-    dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(kp, 64)), unsigned(batch)};
-    // 16-byte loads need an aligned base and 4-element pitches; the packed
-    // pitch kp is a multiple of 8.
-    const bool kc = ks == 1;
-    const int64_t pitch = kc ? rs : ks;
-    const bool vec = aligned16(x) && pitch % 4 == 0 && (batch == 1 || stride % 4 == 0) &&
-                     (rs == 1 || ks == 1);
-    if (vec) {
-#define PB_PACKV(MODE)                                                                      \
-  do {                                                                                      \
-    if (kc)                                                                                 \
-      hipLaunchKernelGGL((pack_vec_kernel<MODE, true>), grid, dim3(256), 0, h->stream, x, rs, \
-                         ks, rows, K, stride, kp, slab, hi, lo, scales, sidx);              \
-    else                                                                                    \
-      hipLaunchKernelGGL((pack_vec_kernel<MODE, false>), grid, dim3(256), 0, h->stream, x,  \
-                         rs, ks, rows, K, stride, kp, slab, hi, lo, scales, sidx);          \
-  } while (0)
-      if (f16s) PB_PACKV(kPackF16Split);
-      else if (split) PB_PACKV(kPackBf16Split);
-      else if (prec == PB_GEMM_BF16) PB_PACKV(kPackBf16);
-      else PB_PACKV(kPackFp8);
-#undef PB_PACKV
-      return;
-    }
-    if (f16s)
-      hipLaunchKernelGGL(pack_kernel<kPackF16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
-                         rows, K, stride, kp, slab, hi, lo, scales, sidx);
-    else if (split)
-      hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
-                         rows, K, stride, kp, slab, hi, lo, scales, sidx);
-    else if (prec == PB_GEMM_BF16)
-      hipLaunchKernelGGL(pack_kernel<kPackBf16>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
-                         stride, kp, slab, hi, lo, scales, sidx);
-    else
-      hipLaunchKernelGGL(pack_kernel<kPackFp8>, grid, dim3(256), 0, h->stream, x, rs, ks, rows, K,
-                         stride, kp, slab, hi, lo, scales, sidx);
-  };
-  pack(g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0);
-  pack(g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1);
-  PB_CHECK_LAUNCH();
 
   const bool c_vec = aligned16(g.c) && g.ldc % 4 == 0 && (batch == 1 || g.sc % 4 == 0);
   if (fp8 && use_mt) {
@@ -842,11 +1021,13 @@ pb_status_t pb_pack_bf16x3(pb_handle_t h, int operand, char trans, int64_t rows,
   const float* none = nullptr;
   constexpr int slab = 16;
   if (vec && ks == 1)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, true>), grid, dim3(256), 0, h->stream, x,
-                       rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0);
+    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, true>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+                       h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0,
+                       static_cast<const int*>(nullptr), static_cast<float*>(nullptr));
   else if (vec)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, false>), grid, dim3(256), 0, h->stream,
-                       x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0);
+    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, false>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+                       h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0,
+                       static_cast<const int*>(nullptr), static_cast<float*>(nullptr));
   else
     hipLaunchKernelGGL(pack_kernel<kPackBf16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
                        rows, k, int64_t(0), ldp, slab, dh, dl, none, 0);
@@ -912,11 +1093,15 @@ pb_status_t pb_pack_fp16x3(pb_handle_t h, int operand, char trans, int64_t rows,
   void* dl = static_cast<void*>(lo);
   constexpr int slab = 16;
   if (vec && ks == 1)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true>), grid, dim3(256), 0, h->stream, x,
-                       rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, static_cast<const float*>(scales), 0);
+    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+                       h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl,
+                       static_cast<const float*>(scales), 0, static_cast<const int*>(nullptr),
+                       static_cast<float*>(nullptr));
   else if (vec)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false>), grid, dim3(256), 0, h->stream,
-                       x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, static_cast<const float*>(scales), 0);
+    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+                       h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl,
+                       static_cast<const float*>(scales), 0, static_cast<const int*>(nullptr),
+                       static_cast<float*>(nullptr));
   else
     hipLaunchKernelGGL(pack_kernel<kPackF16Split>, grid, dim3(256), 0, h->stream, x, rs, ks,
                        rows, k, int64_t(0), ldp, slab, dh, dl, static_cast<const float*>(scales), 0);

@@ -367,6 +367,43 @@ def test_fp16x3_scale_over_every_workgroup_and_finite_only(handle, torch_cuda, p
     assert (err <= tol).all(), (err / tol).max().item()
 
 
+def test_fp16x3_guessed_scales_match_two_pass(torch_cuda):
+    """pb_gemm_ex(FP16X3) packs with the scale exponents of the handle's
+    previous FP16x3 call and packs an operand again when its amax gives
+    another exponent.  On a fresh handle, over a sequence that starts with
+    both guesses wrong, then right, then A's wrong, then B's, then both, every
+    product equals the two-pass pack-once path (amax first) bit for bit."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    h = pb.SB_Handle()
+    m = n = 4096
+    k = 256
+    A0 = make(torch, m * k, 71)
+    B0 = make(torch, k * n, 72)
+    ldp = k
+    ah = torch.empty(m * ldp, dtype=torch.int16, device="cuda")
+    al, bh, bl = torch.empty_like(ah), torch.empty_like(ah), torch.empty_like(ah)
+    fac = torch.zeros(2, device="cuda")
+    seq = [(1.0, 1.0), (1.0, 1.0), (2.0 ** 20, 1.0), (2.0 ** 20, 2.0 ** -30), (1.0, 1.0), (1.0, 1.0)]
+    for i, (fa, fb) in enumerate(seq):
+        A, B = A0 * fa, B0 * fb
+        C1 = torch.zeros(m * n, device="cuda")
+        C2 = torch.zeros(m * n, device="cuda")
+        pb._gemm_ex(h, pb.gemm_precision_t.fp16x3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C1, m)
+        pb.check(L.pb_pack_fp16x3(h.handle, 0, b"n", m, k, A.data_ptr(), m, ah.data_ptr(), al.data_ptr(), ldp,
+                                  fac.data_ptr()))
+        pb.check(L.pb_pack_fp16x3(h.handle, 1, b"n", n, k, B.data_ptr(), k, bh.data_ptr(), bl.data_ptr(), ldp,
+                                  fac.data_ptr() + 4))
+        pb.check(L.pb_gemm_fp16x3_packed(h.handle, m, n, k, 1.0, ah.data_ptr(), al.data_ptr(), fac.data_ptr(),
+                                         ldp, bh.data_ptr(), bl.data_ptr(), fac.data_ptr() + 4, ldp, 0.0,
+                                         C2.data_ptr(), m))
+        torch.cuda.synchronize()
+        assert torch.isfinite(C1).all(), i
+        assert torch.equal(C1, C2), (i, (C1 - C2).abs().max().item())
+
+
 @pytest.mark.parametrize("m,n,k", [(4096, 4096, 512), (300, 200, 100)])
 def test_pack_once_fp16x3_matches_gemm_ex(handle, torch_cuda, m, n, k):
     """pb_pack_fp16x3 + pb_gemm_fp16x3_packed: the factor is 2^-s with
