@@ -481,6 +481,9 @@ inline bool eligible(pb_handle_t h, int64_t M, int64_t N, int64_t batch, bool in
 // NB = 2 (two workgroups per CU) 35.6 / 40.9, NB = 3 33.7 / 37.7, NB = 4
 // 35.0 / 39.3, NB = 5 36.7 / 41.0; the one-problem-per-workgroup kernel
 // 36.7 / 40.6.  Streaming (non-temporal) C stores: 35.0 / 44.7 at NB = 3.
+// Four buffers with two problems in flight (no barrier after a step's reads,
+// since the buffer refilled next was read two steps back): 33.6 / 37.7, the
+// same as three buffers with one barrier more per problem.
 constexpr int kStreamBuffers = 3;
 
 template <typename T>
