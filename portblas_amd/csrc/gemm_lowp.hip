@@ -360,8 +360,8 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
   if constexpr (SC == kScaleRepack)
     if (*ctl == 0) return;
   __shared__ float tile[64][65];  // [row][k]
-  const int64_t r0 = int64_t(blockIdx.x) * 64;
-  const int64_t kt0 = int64_t(blockIdx.y) * kPackKTiles;
+  const int64_t r0 = int64_t(KC ? blockIdx.y : blockIdx.x) * 64;
+  const int64_t kt0 = int64_t(KC ? blockIdx.x : blockIdx.y) * kPackKTiles;
   const int64_t b = blockIdx.z;
   const float* xb = x + b * stride;
   int64_t nkt = (kp + 63) / 64 - kt0;
@@ -489,10 +489,19 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
   }
 }
 
-// Grid of pack_vec_kernel for op(X) with `rows` rows and packed width kp.
-inline dim3 pack_vec_grid(int64_t rows, int64_t kp, int64_t batch) {
-  return dim3{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(pb::ceil_div(kp, 64), kPackKTiles)),
-              unsigned(batch)};
+// Grid of pack_vec_kernel<·, KC> for op(X) with `rows` rows and packed width
+// kp.  Consecutive workgroups take neighbouring tiles along X's contiguous
+// dimension: row tiles on x for M/N-contiguous operands, K tiles on x for
+// K-contiguous ones (KC).
+inline dim3 pack_vec_grid(int64_t rows, int64_t kp, int64_t batch, bool kc) {
+  const unsigned rt = unsigned(pb::ceil_div(rows, 64));
+  const unsigned kt = unsigned(pb::ceil_div(pb::ceil_div(kp, 64), kPackKTiles));
+  return kc ? dim3{kt, rt, unsigned(batch)} : dim3{rt, kt, unsigned(batch)};
+}
+
+// Whether that grid's y extent fits the launch limit.
+inline bool pack_vec_fits(int64_t rows, int64_t kp, bool kc) {
+  return (kc ? pb::ceil_div(rows, 64) : pb::ceil_div(pb::ceil_div(kp, 64), kPackKTiles)) <= 65535;
 }
 
 // ---------------------------------------------------------------------------
@@ -699,9 +708,10 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   PB_REQUIRE(batch <= 65535, "batch too large for the pre-pass");
   // 16-byte loads need an aligned base and 4-element pitches; the packed
   // pitch kp is a multiple of 8.
-  auto vec_ok = [&](const float* x, int64_t rs, int64_t ks, int64_t stride) {
+  auto vec_ok = [&](const float* x, int64_t rs, int64_t ks, int64_t rows, int64_t stride) {
     const int64_t pitch = ks == 1 ? rs : ks;
-    return aligned16(x) && pitch % 4 == 0 && (batch == 1 || stride % 4 == 0) && (rs == 1 || ks == 1);
+    return aligned16(x) && pitch % 4 == 0 && (batch == 1 || stride % 4 == 0) && (rs == 1 || ks == 1) &&
+           pack_vec_fits(rows, kp, ks == 1);
   };
   // FP16x3 with both operands on the vectorized pre-pass skips the amax
   // passes: each operand is packed with the scale exponent of the handle's
@@ -709,8 +719,8 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
   // sets the true scales and flags an operand whose exponent changed, and
   // that operand alone is packed again (the other repack launch exits at
   // once).  The packed planes and scales are those of the two-pass order.
-  const bool guess = f16s && vec_ok(g.a, a_rs, a_ks, g.sa) && vec_ok(g.b, b_rs, b_ks, g.sb);
-  const dim3 ga = pack_vec_grid(M, kp, batch), gb = pack_vec_grid(N, kp, batch);
+  const bool guess = f16s && vec_ok(g.a, a_rs, a_ks, M, g.sa) && vec_ok(g.b, b_rs, b_ks, N, g.sb);
+  const dim3 ga = pack_vec_grid(M, kp, batch, a_ks == 1), gb = pack_vec_grid(N, kp, batch, b_ks == 1);
   const int64_t gna = int64_t(ga.x) * ga.y * ga.z, gnb = int64_t(gb.x) * gb.y * gb.z;
   PB_REQUIRE(!guess || (gna <= INT32_MAX && gnb <= INT32_MAX), "operand too large for the pre-pass");
   const size_t guess_bytes = guess ? pb::align_up(4 * size_t(gna), 256) + pb::align_up(4 * size_t(gnb), 256) + 256 : 0;
@@ -776,8 +786,8 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
       const bool kc = ks == 1;
       const int* none_i = nullptr;
       float* none_f = nullptr;
-      if (vec_ok(x, rs, ks, stride)) {
-        const dim3 vgrid = pack_vec_grid(rows, kp, batch);
+      if (vec_ok(x, rs, ks, rows, stride)) {
+        const dim3 vgrid = pack_vec_grid(rows, kp, batch, kc);
 #define PB_PACKV(MODE)                                                                       \
     do {                                                                                       \
       if (kc)                                                                                  \
@@ -1015,17 +1025,18 @@ pb_status_t pb_pack_bf16x3(pb_handle_t h, int operand, char trans, int64_t rows,
   else { rs = t == 'n' ? ldx : 1; ks = t == 'n' ? 1 : ldx; }
   PB_REQUIRE(ldx >= ((rs == 1) ? rows : k), "leading dimension too small");
   dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(ldp, 64)), 1u};
-  const bool vec = aligned16(x) && (rs == 1 ? ks : rs) % 4 == 0 && (rs == 1 || ks == 1);
+  const bool vec = aligned16(x) && (rs == 1 ? ks : rs) % 4 == 0 && (rs == 1 || ks == 1) &&
+                   pack_vec_fits(rows, ldp, ks == 1);
   void* dh = static_cast<void*>(hi);
   void* dl = static_cast<void*>(lo);
   const float* none = nullptr;
   constexpr int slab = 16;
   if (vec && ks == 1)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, true>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, true>), pack_vec_grid(rows, ldp, 1, true), dim3(256), 0,
                        h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0,
                        static_cast<const int*>(nullptr), static_cast<float*>(nullptr));
   else if (vec)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, false>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+    hipLaunchKernelGGL((pack_vec_kernel<kPackBf16Split, false>), pack_vec_grid(rows, ldp, 1, false), dim3(256), 0,
                        h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl, none, 0,
                        static_cast<const int*>(nullptr), static_cast<float*>(nullptr));
   else
@@ -1088,17 +1099,18 @@ pb_status_t pb_pack_fp16x3(pb_handle_t h, int operand, char trans, int64_t rows,
   hipLaunchKernelGGL(f16_scales_kernel, dim3(1), dim3(kAmaxBlock), 0, h->stream, part, na,
                      static_cast<const float*>(nullptr), 0, scales, factor);
   dim3 grid{unsigned(pb::ceil_div(rows, 64)), unsigned(pb::ceil_div(ldp, 64)), 1u};
-  const bool vec = aligned16(x) && (rs == 1 ? ks : rs) % 4 == 0 && (rs == 1 || ks == 1);
+  const bool vec = aligned16(x) && (rs == 1 ? ks : rs) % 4 == 0 && (rs == 1 || ks == 1) &&
+                   pack_vec_fits(rows, ldp, ks == 1);
   void* dh = static_cast<void*>(hi);
   void* dl = static_cast<void*>(lo);
   constexpr int slab = 16;
   if (vec && ks == 1)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true>), pack_vec_grid(rows, ldp, 1, true), dim3(256), 0,
                        h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl,
                        static_cast<const float*>(scales), 0, static_cast<const int*>(nullptr),
                        static_cast<float*>(nullptr));
   else if (vec)
-    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false>), pack_vec_grid(rows, ldp, 1), dim3(256), 0,
+    hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false>), pack_vec_grid(rows, ldp, 1, false), dim3(256), 0,
                        h->stream, x, rs, ks, rows, k, int64_t(0), ldp, slab, dh, dl,
                        static_cast<const float*>(scales), 0, static_cast<const int*>(nullptr),
                        static_cast<float*>(nullptr));

@@ -146,3 +146,58 @@ def test_set_stream_orders_after_old_stream(torch_cuda):
         got = C.cpu().numpy()
         assert np.all(np.abs(got - ref) <= 1.01 * (k + 2) * U32 * ap + 1e-30)
     hs.close()
+
+
+def test_fp16x3_graph_replay_follows_scale_changes(torch_cuda):
+    """An FP16x3 call packs with the scale exponents of the handle's previous
+    FP16x3 call and packs again when they change.  Replays of a captured call
+    must follow changes of the input's magnitude between replays: each replay
+    equals the eager product of the current inputs, and scaling A by 2^12
+    scales C by exactly 2^12."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    L = pb.load()
+    s = torch.cuda.Stream()
+    hs = pb.SB_Handle(s)
+    m = n = 4096
+    k = 256
+    g = torch.Generator(device="cuda").manual_seed(13)
+    A = torch.rand(m * k, device="cuda", generator=g) * 2 - 1
+    B = torch.rand(k * n, device="cuda", generator=g) * 2 - 1
+    Ab = A.clone()
+    C = torch.zeros(m * n, device="cuda")
+    P = pb.gemm_precision_t.fp16x3
+
+    def call():
+        pb._gemm_ex(hs, P, "n", "n", m, n, k, 1.0, Ab, m, B, k, 0.0, C, m)
+
+    with torch.cuda.stream(s):
+        call()
+    hs.wait()
+    C1 = C.clone()
+    Ab.mul_(4096.0)
+    torch.cuda.synchronize()
+    with torch.cuda.stream(s):
+        call()
+    hs.wait()
+    C2 = C.clone()
+    assert torch.equal(C2, C1 * 4096.0)
+    graph = ctypes.c_void_p()
+    with torch.cuda.stream(s):
+        pb.check(L.pb_graph_capture_begin(hs.handle))
+        call()
+        pb.check(L.pb_graph_capture_end(hs.handle, ctypes.byref(graph)))
+    try:
+        for rep, (scale, want) in enumerate([(1.0, C1), (4096.0, C2), (4096.0, C2), (1.0, C1), (1.0, C1)]):
+            Ab.copy_(A * scale)
+            C.fill_(float("nan"))
+            torch.cuda.synchronize()
+            pb.check(L.pb_graph_launch(hs.handle, graph))
+            hs.wait()
+            z = torch.ones(1 << 20, device="cuda").sum()  # eager work between replays
+            torch.cuda.synchronize()
+            assert z.item() == float(1 << 20)
+            assert torch.equal(C, want), f"replay {rep} differs"
+    finally:
+        pb.check(L.pb_graph_destroy(graph))
