@@ -236,15 +236,15 @@ void small_gemm_kernel(GemmProblem<T> p) {
   store_c<T>(p, p.c + b * p.c_stride, M, N, wm, wn, lane, acc);
 }
 
-// FP32 64 x 64 problems with K a multiple of 64 and vector-aligned operands
-// move every K chunk of 64 from HBM straight into LDS (global_load_lds, 16
-// bytes a lane, no register staging and no LDS store instructions): 16 KB per
-// operand as 64 rows of 256 bytes.  An operand keeps its contiguous dimension
+// FP32 64^3 problems with vector-aligned operands move their operands from
+// HBM straight into LDS (global_load_lds, 16 bytes a lane, no register
+// staging and no LDS store instructions): 16 KB per operand as 64 rows of
+// 256 bytes.  An operand keeps its contiguous dimension
 // along the LDS row: M/N-contiguous ones as [k][r], K-contiguous ones (KC) as
 // [r][k].  The 16-byte chunk c of row rho sits at slot c ^ swz(rho); the DMA
 // applies the swizzle by the choice of global chunk each lane fetches.
 //
-// MFMA step s (0..15) of a chunk takes k = 16 (s / 4) + 4 g + s % 4 from lane
+// MFMA step s (0..15) of a problem takes k = 16 (s / 4) + 4 g + s % 4 from lane
 // group g = lane / 16, so a [r][k] operand reads the 4 k of 4 consecutive
 // steps with one 16-byte read (swz = row & 15: the 16 lanes of a group read 16
 // different slots) and a [k][r] operand reads one element per step (swz =
@@ -254,14 +254,13 @@ __device__ __forceinline__ int dma_swz(int rho) {
   return KC ? (rho & 15) : (((rho >> 2) & 3) << 2);
 }
 
-// One wave's share of a chunk's DMA: waves 0, 1 load rows 32 h .. 32 h + 31
+// One wave's share of a problem's DMA: waves 0, 1 load rows 32 h .. 32 h + 31
 // of A, waves 2, 3 of B; DMA j covers 4 rows (1 KB), lane l row 4 j + l / 16,
 // slot l % 16.
 template <bool KCA, bool KCB>
 struct DmaShare {
   int op, h;
-  int64_t src[8];  // element offset of the lane's chunk in K chunk 0
-  int64_t kstep;   // element offset of the next K chunk
+  int64_t src[8];  // element offset of the lane's 16-byte chunk in the operand
 
   __device__ __forceinline__ DmaShare(const GemmProblem<float>& p, int wave, int lane) {
     op = wave >> 1;
@@ -276,7 +275,6 @@ struct DmaShare {
       const int c = (lane & 15) ^ (kc ? dma_swz<true>(rho) : dma_swz<false>(rho));
       src[j] = int64_t(rho) * pitch + 4 * c;
     }
-    kstep = kc ? 64 : 64 * ks;
   }
 
   // Problem q's operand base for this wave.
@@ -284,20 +282,19 @@ struct DmaShare {
     return op == 0 ? p.a[0] + q * p.a_stride : p.b[0] + q * p.b_stride;
   }
 
-  // Issues the wave's 8 DMAs of K chunk kc of operand X into `lds`
-  // (A at lds[0 .. 4095], B at lds[4096 .. 8191]).
-  __device__ __forceinline__ void issue(const float* X, int64_t kc, float* lds) const {
+  // Issues the wave's 8 DMAs of operand X into `lds` (A at lds[0 .. 4095],
+  // B at lds[4096 .. 8191]).
+  __device__ __forceinline__ void issue(const float* X, float* lds) const {
     float* dst = lds + op * kTile * kTile + 32 * h * kTile;
-    const int64_t kofs = kc * kstep;
 #pragma unroll
     for (int j = 0; j < 8; ++j)
-      __builtin_amdgcn_global_load_lds(pbdev::to_global(X + src[j] + kofs),
+      __builtin_amdgcn_global_load_lds(pbdev::to_global(X + src[j]),
                                        (__attribute__((address_space(3))) void*)(dst + 4 * j * kTile),
                                        16, 0, 0);
   }
 };
 
-// The 64 MFMAs of one chunk for the wave's 32 x 32 quarter (wm, wn).
+// The 64 MFMAs of one problem for the wave's 32 x 32 quarter (wm, wn).
 template <bool KCA, bool KCB>
 struct DmaMfma {
   int aoff[2], boff[2], g, wm, wn, r16;
@@ -410,7 +407,7 @@ __global__ __launch_bounds__(256) void small_stream_kernel(GemmProblem<float> p)
     // __syncthreads() would also wait for every DMA in flight (its fence
     // drains the vector memory counter), so the barriers are written out.
     if (it + D < n) {
-      d.issue(d.base(p, w + (it + D) * G), 0, buf(std::integral_constant<int, RN>{}));
+      d.issue(d.base(p, w + (it + D) * G), buf(std::integral_constant<int, RN>{}));
       if (it < D) {  // no stores of problems before it + 1 - D yet: 8 D or more after it
         if constexpr (D == 1) asm volatile("s_waitcnt vmcnt(8)\n\ts_barrier" ::: "memory");
         else if constexpr (D == 2) asm volatile("s_waitcnt vmcnt(16)\n\ts_barrier" ::: "memory");
@@ -447,7 +444,7 @@ __global__ __launch_bounds__(256) void small_stream_kernel(GemmProblem<float> p)
   auto prologue = [&](auto r) {
     constexpr int R = decltype(r)::value;
     if constexpr (R < D)
-      if (R < n) d.issue(d.base(p, w + R * G), 0, buf(r));
+      if (R < n) d.issue(d.base(p, w + R * G), buf(r));
   };
   using I0 = std::integral_constant<int, 0>;
   using I1 = std::integral_constant<int, 1>;
