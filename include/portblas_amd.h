@@ -66,7 +66,8 @@ typedef enum pb_gemm_precision_t {
   PB_GEMM_FP32_SIMT = 1, /* plain FP32 on vector FMA, LDS-tiled */
   PB_GEMM_BF16X3 = 2,    /* split precision: hi*hi + hi*lo + lo*hi on BF16 MFMA, FP32 accumulate */
   PB_GEMM_BF16 = 3,      /* inputs rounded to BF16, BF16 MFMA, FP32 accumulate */
-  PB_GEMM_FP8_E4M3 = 4,  /* inputs scaled by per-tensor amax to OCP e4m3, FP8 MFMA, FP32 accumulate */
+  PB_GEMM_FP8_E4M3 = 4,  /* inputs scaled by per-tensor powers of two (amax to at most 448) to OCP e4m3,
+                          * FP8 MFMA, FP32 accumulate */
   PB_GEMM_BF16X4 = 5,    /* BF16X3 plus the lo*lo term: smaller representation error */
   PB_GEMM_FP16X3 = 6     /* split precision in IEEE half: each operand scaled by a power of two
                           * (amax over its finite elements to [2^14, 2^15)), then hi = f16(x),
@@ -174,8 +175,8 @@ pb_status_t pb_dgemm_strided_batched(pb_handle_t h, char transa, char transb,
 /* FP32-in / FP32-out GEMM with a selectable arithmetic (no reference
  * counterpart beyond blas::_gemm itself; the reduced-precision precedent is
  * src/operations/blas3/gemm_local_joint_matrix.hpp).  Same semantics as
- * pb_sgemm_strided_batched.  PB_GEMM_FP8_E4M3 derives one scale per operand
- * from amax over all of that operand's batches. */
+ * pb_sgemm_strided_batched.  PB_GEMM_FP8_E4M3 derives one power-of-two scale
+ * per operand from amax over all of that operand's batches. */
 pb_status_t pb_gemm_ex(pb_handle_t h, pb_gemm_precision_t precision, char transa,
                        char transb, int64_t m, int64_t n, int64_t k, float alpha,
                        const float* a, int64_t lda, int64_t stridea, const float* b,

@@ -154,18 +154,29 @@ __device__ __forceinline__ void fold_amax(const float* pa, int na, const float* 
   amax[1] = red[1][0];
 }
 
-// FP8 scales: scales[0] = 448 / amax(A), scales[1] = 448 / amax(B) (1 when
-// the amax is 0 or not finite), scales[2] = 1 / (scales[0] scales[1]).
+// Power-of-two scale exponent of one FP8 e4m3 operand: the largest s with
+// amax * 2^s <= 448 (e4m3's largest finite value), so no element saturates
+// and the scaling itself is exact.  0 when the amax is 0 or not finite;
+// clamped to [-63, 63] so that 2^-(s(A) + s(B)) stays a normal float.
+__device__ __forceinline__ int fp8_scale_exp(float amax) {
+  if (!(amax > 0.f && amax < __builtin_huge_valf())) return 0;
+  int e;
+  const float f = frexpf(amax, &e);  // amax = f 2^e, f in [0.5, 1); 448 = 0.875 2^9
+  const int s = (f <= 0.875f ? 9 : 8) - e;
+  return s < -63 ? -63 : (s > 63 ? 63 : s);
+}
+
+// FP8 scales: scales[0] = 2^s(A), scales[1] = 2^s(B), scales[2] =
+// 2^-(s(A) + s(B)).
 __global__ __launch_bounds__(kAmaxBlock) void fp8_scales_kernel(const float* pa, int na, const float* pb, int nb,
                                                                 float* scales) {
   float am[2];
   fold_amax(pa, na, pb, nb, am);
   if (threadIdx.x != 0) return;
-  const float sa = (am[0] > 0.f && am[0] < 3.0e38f) ? 448.0f / am[0] : 1.0f;
-  const float sb = (am[1] > 0.f && am[1] < 3.0e38f) ? 448.0f / am[1] : 1.0f;
-  scales[0] = sa;
-  scales[1] = sb;
-  scales[2] = (1.0f / sa) * (1.0f / sb);
+  const int ea = fp8_scale_exp(am[0]), eb = fp8_scale_exp(am[1]);
+  scales[0] = ldexpf(1.0f, ea);
+  scales[1] = ldexpf(1.0f, eb);
+  scales[2] = ldexpf(1.0f, -(ea + eb));
 }
 
 // Power-of-two scale exponent of one FP16x3 operand: amax * 2^s in
@@ -196,16 +207,17 @@ __global__ __launch_bounds__(kAmaxBlock) void f16_scales_kernel(const float* pa,
   if (factor) factor[0] = ldexpf(1.0f, -ea);
 }
 
-// FP16x3 scales from packs that used guessed exponents guess[0..1]: writes
-// scales as f16_scales_kernel does, flags[i] = 1 where the true exponent
-// differs from the guess (that operand is packed again), and the true
-// exponents into guess for the next call.
+// FP16x3 (FP8: FP8) scales from packs that used guessed exponents
+// guess[0..1]: writes scales as f16_scales_kernel (fp8_scales_kernel) does,
+// flags[i] = 1 where the true exponent differs from the guess (that operand
+// is packed again), and the true exponents into guess for the next call.
 // One workgroup of kCheckBlock threads, 8 independent loads in flight per
 // thread and operand (the packs leave one partial per workgroup: 8192 per
 // 8192^2 operand).
 constexpr int kCheckBlock = 1024;
-__global__ __launch_bounds__(kCheckBlock) void f16_check_kernel(const float* pa, int na, const float* pb, int nb,
-                                                                int* guess, float* scales, int* flags) {
+template <bool FP8>
+__global__ __launch_bounds__(kCheckBlock) void scale_check_kernel(const float* pa, int na, const float* pb, int nb,
+                                                                  int* guess, float* scales, int* flags) {
   __shared__ float red[2][kCheckBlock / 64];
   constexpr int U = 8;
   float m[2][U];
@@ -242,7 +254,8 @@ __global__ __launch_bounds__(kCheckBlock) void f16_check_kernel(const float* pa,
     am[0] = fmaxf(am[0], red[0][w]);
     am[1] = fmaxf(am[1], red[1][w]);
   }
-  const int ea = f16_scale_exp(am[0]), eb = f16_scale_exp(am[1]);
+  const int ea = FP8 ? fp8_scale_exp(am[0]) : f16_scale_exp(am[0]);
+  const int eb = FP8 ? fp8_scale_exp(am[1]) : f16_scale_exp(am[1]);
   scales[0] = ldexpf(1.0f, ea);
   scales[1] = ldexpf(1.0f, eb);
   scales[2] = ldexpf(1.0f, -(ea + eb));
@@ -446,7 +459,7 @@ __global__ __launch_bounds__(256) void pack_vec_kernel(const float* __restrict__
     for (int e = 0; e < 16; ++e) {
       const float t = tile[rr][kq + e];
       // Every element of op(X) is converted by exactly one thread.
-      if constexpr (SC == kScaleGuess) best = fmaxf(best, amag<true>(t));
+      if constexpr (SC == kScaleGuess) best = fmaxf(best, amag<MODE == kPackF16Split>(t));
       pack_convert<MODE>(t * s, h[e], l[e], q[e]);
     }
     if constexpr (MODE == kPackFp8) {
@@ -713,13 +726,14 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
     return aligned16(x) && pitch % 4 == 0 && (batch == 1 || stride % 4 == 0) && (rs == 1 || ks == 1) &&
            pack_vec_fits(rows, kp, ks == 1);
   };
-  // FP16x3 with both operands on the vectorized pre-pass skips the amax
-  // passes: each operand is packed with the scale exponent of the handle's
-  // previous FP16x3 call while the pack takes its amax; a check kernel then
-  // sets the true scales and flags an operand whose exponent changed, and
-  // that operand alone is packed again (the other repack launch exits at
-  // once).  The packed planes and scales are those of the two-pass order.
-  const bool guess = f16s && vec_ok(g.a, a_rs, a_ks, M, g.sa) && vec_ok(g.b, b_rs, b_ks, N, g.sb);
+  // FP16x3 and FP8 with both operands on the vectorized pre-pass skip the
+  // amax passes: each operand is packed with the scale exponent of the
+  // handle's previous call of the same precision while the pack takes its
+  // amax; a check kernel then sets the true scales and flags an operand whose
+  // exponent changed, and that operand alone is packed again (the other
+  // repack launch exits at once).  The packed planes and scales are those of
+  // the two-pass order.
+  const bool guess = (f16s || fp8) && vec_ok(g.a, a_rs, a_ks, M, g.sa) && vec_ok(g.b, b_rs, b_ks, N, g.sb);
   const dim3 ga = pack_vec_grid(M, kp, batch, a_ks == 1), gb = pack_vec_grid(N, kp, batch, b_ks == 1);
   const int64_t gna = int64_t(ga.x) * ga.y * ga.z, gnb = int64_t(gb.x) * gb.y * gb.z;
   PB_REQUIRE(!guess || (gna <= INT32_MAX && gnb <= INT32_MAX), "operand too large for the pre-pass");
@@ -740,23 +754,33 @@ pb_status_t gemm_ex_lowp(pb_handle_t h, pb_gemm_precision_t prec, const GemmCall
     float* pbp = sc.take<float>(size_t(gnb));
     int* flags = sc.take<int>(2);
     const bool kca = a_ks == 1, kcb = b_ks == 1;
+    int* state = h->scale_guess + (fp8 ? 2 : 0);  // this precision's exponents (A, B)
     auto packg = [&](auto sc_mode, bool kc, dim3 grid, const float* x, int64_t rs, int64_t ks, int64_t rows,
                      int64_t stride, void* hi, void* lo, int sidx, const int* ctl, float* part) {
       constexpr int SCM = decltype(sc_mode)::value;
-      if (kc)
-        hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, true, SCM>), grid, dim3(256), 0, h->stream, x, rs, ks,
-                           rows, K, stride, kp, slab, hi, lo, static_cast<const float*>(scales), sidx, ctl, part);
-      else
-        hipLaunchKernelGGL((pack_vec_kernel<kPackF16Split, false, SCM>), grid, dim3(256), 0, h->stream, x, rs,
-                           ks, rows, K, stride, kp, slab, hi, lo, static_cast<const float*>(scales), sidx, ctl,
-                           part);
+      const float* sp = scales;
+#define PB_PACKG(MODE, KC)                                                                                      \
+  hipLaunchKernelGGL((pack_vec_kernel<MODE, KC, SCM>), grid, dim3(256), 0, h->stream, x, rs, ks, rows, K, stride, \
+                     kp, slab, hi, lo, sp, sidx, ctl, part)
+      if (fp8) {
+        if (kc) PB_PACKG(kPackFp8, true);
+        else PB_PACKG(kPackFp8, false);
+      } else {
+        if (kc) PB_PACKG(kPackF16Split, true);
+        else PB_PACKG(kPackF16Split, false);
+      }
+#undef PB_PACKG
     };
     using Guess = std::integral_constant<int, kScaleGuess>;
     using Repack = std::integral_constant<int, kScaleRepack>;
-    packg(Guess{}, kca, ga, g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0, h->f16_guess, pa);
-    packg(Guess{}, kcb, gb, g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1, h->f16_guess + 1, pbp);
-    hipLaunchKernelGGL(f16_check_kernel, dim3(1), dim3(kCheckBlock), 0, h->stream, pa, int(gna), pbp, int(gnb),
-                       h->f16_guess, scales, flags);
+    packg(Guess{}, kca, ga, g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0, state, pa);
+    packg(Guess{}, kcb, gb, g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1, state + 1, pbp);
+    if (fp8)
+      hipLaunchKernelGGL(scale_check_kernel<true>, dim3(1), dim3(kCheckBlock), 0, h->stream, pa, int(gna), pbp,
+                         int(gnb), state, scales, flags);
+    else
+      hipLaunchKernelGGL(scale_check_kernel<false>, dim3(1), dim3(kCheckBlock), 0, h->stream, pa, int(gna), pbp,
+                         int(gnb), state, scales, flags);
     packg(Repack{}, kca, ga, g.a, a_rs, a_ks, M, g.sa, a_hi, a_lo, 0, flags, static_cast<float*>(nullptr));
     packg(Repack{}, kcb, gb, g.b, b_rs, b_ks, N, g.sb, b_hi, b_lo, 1, flags + 1, static_cast<float*>(nullptr));
     PB_CHECK_LAUNCH();

@@ -130,14 +130,14 @@ pb_status_t pb_handle_create(void* stream, pb_handle_t* out) {
   if (e == hipSuccess)
     e = hipMemset(h->counters, 0, kCounterSlots * sizeof(unsigned int));
   if (e == hipSuccess) e = hipMalloc(&h->scalar_slot, 256);
-  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->f16_guess), 256);
-  if (e == hipSuccess) e = hipMemset(h->f16_guess, 0, 256);
+  if (e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->scale_guess), 256);
+  if (e == hipSuccess) e = hipMemset(h->scale_guess, 0, 256);
   if (e == hipSuccess) e = hipEventCreateWithFlags(&h->switch_event, hipEventDisableTiming);
   if (e == hipSuccess) e = hipDeviceSynchronize();
   if (e != hipSuccess) {
     if (h->counters) (void)hipFree(h->counters);
     if (h->scalar_slot) (void)hipFree(h->scalar_slot);
-    if (h->f16_guess) (void)hipFree(h->f16_guess);
+    if (h->scale_guess) (void)hipFree(h->scale_guess);
     if (h->switch_event) (void)hipEventDestroy(h->switch_event);
     delete h;
     return set_error(PB_STATUS_DEVICE_ERROR, "counter arena setup failed: %s",
@@ -155,7 +155,7 @@ pb_status_t pb_handle_destroy(pb_handle_t h) {
   h->ws.reset();  // freed now unless a live graph still holds it
   if (h->counters) (void)hipFree(h->counters);
   if (h->scalar_slot) (void)hipFree(h->scalar_slot);
-  if (h->f16_guess) (void)hipFree(h->f16_guess);
+  if (h->scale_guess) (void)hipFree(h->scale_guess);
   if (h->switch_event) (void)hipEventDestroy(h->switch_event);
   delete h;
   if (e != hipSuccess)

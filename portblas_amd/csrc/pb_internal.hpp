@@ -99,7 +99,7 @@ struct pb_handle_s {
   std::shared_ptr<pb_device_buffer> ws;
   unsigned int* counters = nullptr;  // kCounterSlots zero-initialised slots
   void* scalar_slot = nullptr;       // 256 bytes for the _sync BLAS-1 results
-  int* f16_guess = nullptr;          // FP16x3 scale exponents of the last call (A, B), device side
+  int* scale_guess = nullptr;        // scale exponents of the last FP16x3 call (A, B) and FP8 call (A, B), device side
   hipEvent_t switch_event = nullptr; // orders a stream switch after the old stream's work
   size_t counter_cursor = 0;
   bool capturing = false;

@@ -6,8 +6,9 @@ reference, with forward-error bounds derived from each format's rounding:
           kept products miss hi*r_b + r_a*hi + lo*lo (<= 3.1 * 2^-16 |a||b|)
   BF16x4  also keeps lo*lo                          (<= 2.1 * 2^-16 |a||b|)
   BF16    |ab - hi_a hi_b| <= (2 * 2^-8 + 2^-16) |a||b|
-  FP8     e4m3 after scaling by 448 / amax: relative 2^-4 plus an absolute
-          2^-10 * amax / 448 below the normal range
+  FP8     e4m3 after scaling by 2^s, the largest power of two with
+          amax 2^s <= 448: relative 2^-4 plus an absolute 2^-10 2^-s below
+          the normal range
   FP16x3  x_s = x 2^s (amax 2^s in [2^14, 2^15)) = hi + lo + r in binary16,
           |x_s - hi| <= 2^-11 |x_s|, |r| <= 2^-22 |x_s| while lo is normal,
           else an absolute 2^-25; the kept products miss hi*r_b + r_a*hi +
@@ -75,7 +76,7 @@ def check_lowp(handle, torch, prec, ta, tb, m, n, k, alpha=1.0, beta=0.0, batch=
         tol = (rep + 1.01 * (terms * k + 2) * U) * abs(alpha) * absprod + 2 * U * abs(beta) * c0.abs()
         if prec in (P.fp8_e4m3, P.fp16x3):
             if prec == P.fp8_e4m3:
-                ea, eb = 2.0 ** -10 * amax_a / 448, 2.0 ** -10 * amax_b / 448
+                ea, eb = 2.0 ** -10 / fp8_scale(amax_a), 2.0 ** -10 / fp8_scale(amax_b)
             else:
                 ea, eb = 2.0 ** -25 / f16_scale(amax_a), 2.0 ** -25 / f16_scale(amax_b)
             sub = 1.07 * (ea * ob.abs().sum(0, keepdim=True) + eb * oa.abs().sum(1, keepdim=True)) + k * ea * eb
@@ -95,6 +96,20 @@ def f16_scale(amax):
         return 1.0
     s = 15 - math.frexp(amax)[1]
     return 2.0 ** min(max(s, -63), 74)
+
+
+def fp8_scale(amax):
+    """The FP8 pre-pass scale: the largest power of two 2^s with
+    amax 2^s <= 448 (e4m3's largest finite value), s in [-63, 63]; 1 for an
+    amax of 0 or infinity."""
+    if not 0 < amax < math.inf:
+        return 1.0
+    s = math.floor(math.log2(448.0 / amax))
+    while amax * 2.0 ** (s + 1) <= 448.0:
+        s += 1
+    while amax * 2.0 ** s > 448.0:
+        s -= 1
+    return 2.0 ** min(max(s, -63), 63)
 
 
 PRECS = ["bf16x3", "bf16x4", "bf16", "fp8_e4m3", "fp16x3"]
@@ -151,8 +166,8 @@ def test_lowp_wide_range_inputs(handle, torch_cuda, dist):
 ])
 def test_lowp_exact_on_integer_inputs(handle, torch_cuda, prec, ta, tb, m, n, k, batch):
     """Integers in [-2, 2] are exact in every format (BF16 hi with lo = 0;
-    FP8 after the per-tensor scale 448 / 2 = 224, which maps them to 0, 224,
-    448), their products and FP32 sums are exact, so every K step must land
+    FP8 after the per-tensor scale 2^7, which maps them to 0, 128, 256),
+    their products and FP32 sums are exact, so every K step must land
     exactly once: the result is the integer product up to the final
     dequantising multiply (a step dropped, repeated or read from the wrong
     ring stage moves it by whole integers)."""
@@ -185,7 +200,7 @@ def test_fp8_amax_over_every_workgroup(handle, torch_cuda, pos):
     (1000, against U[-1,1] elsewhere) at the first or last element of a
     16.8M-element operand, which the first or last of the 2048 workgroups
     sees, must set the scale (a missed partial would saturate the needle at
-    448 / amax(rest) and break the error bound)."""
+    448 2^-s(rest) and break the error bound)."""
     import portblas_amd as pb
 
     torch = torch_cuda
@@ -202,7 +217,7 @@ def test_fp8_amax_over_every_workgroup(handle, torch_cuda, pos):
     oa, ob = cm(A, m, k, m).double(), cm(B, k, n, k).double()
     ref = oa @ ob
     absprod = oa.abs() @ ob.abs()
-    ea = eb = 2.0 ** -10 * 1000.0 / 448
+    ea = eb = 2.0 ** -10 / fp8_scale(1000.0)
     tol = (0.13 + 1.01 * (k + 2) * U) * absprod + 1.07 * (ea * ob.abs().sum(0, keepdim=True)
                                                        + eb * oa.abs().sum(1, keepdim=True)) + k * ea * eb
     err = (cm(C, m, n, m).double() - ref).abs()
@@ -399,6 +414,32 @@ def test_fp16x3_guessed_scales_match_two_pass(torch_cuda):
         pb.check(L.pb_gemm_fp16x3_packed(h.handle, m, n, k, 1.0, ah.data_ptr(), al.data_ptr(), fac.data_ptr(),
                                          ldp, bh.data_ptr(), bl.data_ptr(), fac.data_ptr() + 4, ldp, 0.0,
                                          C2.data_ptr(), m))
+        torch.cuda.synchronize()
+        assert torch.isfinite(C1).all(), i
+        assert torch.equal(C1, C2), (i, (C1 - C2).abs().max().item())
+
+
+def test_fp8_guessed_scales_match_fresh_handle(torch_cuda):
+    """pb_gemm_ex(FP8) packs with the power-of-two scales of the handle's
+    previous FP8 call and packs an operand again when its amax gives another
+    one.  Over a sequence of hits and misses on one handle, every product
+    equals the product on a fresh handle (whose first call always takes the
+    scales from the amax it measures), bit for bit."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    h = pb.SB_Handle()
+    m = n = 4096
+    k = 256
+    A0 = make(torch, m * k, 81)
+    B0 = make(torch, k * n, 82)
+    seq = [(1.0, 1.0), (1.0, 1.0), (2.0 ** 20, 1.0), (2.0 ** 20, 2.0 ** -30), (3.0, 1.0), (1.0, 1.0)]
+    for i, (fa, fb) in enumerate(seq):
+        A, B = A0 * fa, B0 * fb
+        C1 = torch.zeros(m * n, device="cuda")
+        C2 = torch.zeros(m * n, device="cuda")
+        pb._gemm_ex(h, pb.gemm_precision_t.fp8_e4m3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C1, m)
+        pb._gemm_ex(pb.SB_Handle(), pb.gemm_precision_t.fp8_e4m3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C2, m)
         torch.cuda.synchronize()
         assert torch.isfinite(C1).all(), i
         assert torch.equal(C1, C2), (i, (C1 - C2).abs().max().item())
