@@ -57,10 +57,11 @@ def parse():
     ap.add_argument("--quick", action="store_true", help="headline + roofline only")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--pmc-probe", nargs="?", const="fp16x3",
-                    choices=("fp16x3", "bf16x3", "dgemm", "grouped", "simt"), default=None,
+                    choices=("fp16x3", "bf16x3", "dgemm", "grouped", "simt", "strided"), default=None,
                     help="for rocprofv3 --pmc: one 256 MiB device copy (the counter scale), then two "
                          "launches of the FP16x3 (or BF16x3) kernel on packed 8192^3 operands (or of DGEMM 3072^3, "
-                         "the grouped set, or the FP32 SIMT GEMM at 4096^3); prints no JSON")
+                         "the grouped set, the FP32 SIMT GEMM at 4096^3, or the strided 4096 x 64^3 batch); "
+                         "prints no JSON")
     return ap.parse_args()
 
 
@@ -109,7 +110,8 @@ def pmc_probe(torch, pb, h, which="fp16x3"):
     DGEMM 3072^3, small enough that SQ_VALU_MFMA_BUSY_CYCLES, summed over
     the 1024 SIMDs, stays below its 2^32 limit; or the 512-problem grouped
     set; or the FP32 SIMT GEMM at 4096^3, whose FP32 FMA instruction count
-    tells packed from non-packed FMAs), twice."""
+    tells packed from non-packed FMAs; or the strided 4096 x 64^3 batch),
+    twice."""
     src = torch.ones(1 << 26, device="cuda")
     dst = torch.empty_like(src)
     torch.cuda.synchronize()
@@ -123,6 +125,17 @@ def pmc_probe(torch, pb, h, which="fp16x3"):
         Cd = torch.zeros(D * D, dtype=torch.float64, device="cuda")
         for _ in range(2):
             pb._gemm(h, "n", "n", D, D, D, 1.0, Ad, D, Bd, D, 0.0, Cd, D)
+            torch.cuda.synchronize()
+        log("pmc probe done")
+        return
+    if which == "strided":
+        bs, sz = 4096, 64
+        Ab = torch.rand(bs * sz * sz, device="cuda") * 2 - 1
+        Bb = torch.rand(bs * sz * sz, device="cuda") * 2 - 1
+        Cb = torch.zeros(bs * sz * sz, device="cuda")
+        for _ in range(2):
+            pb._gemm_strided_batched(h, "n", "n", sz, sz, sz, 1.0, Ab, sz, sz * sz, Bb, sz, sz * sz, 0.0, Cb, sz,
+                                     sz * sz, bs)
             torch.cuda.synchronize()
         log("pmc probe done")
         return
