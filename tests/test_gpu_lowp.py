@@ -419,6 +419,26 @@ def test_fp16x3_guessed_scales_match_two_pass(torch_cuda):
         assert torch.equal(C1, C2), (i, (C1 - C2).abs().max().item())
 
 
+@pytest.mark.parametrize("m,n,k", [(4096, 4096, 512), (300, 200, 520)])
+def test_fp8_power_of_two_scale_exact_on_integers(handle, torch_cuda, m, n, k):
+    """The FP8 scale is the largest power of two 2^s with amax 2^s <= 448:
+    integers in [-3, 3] (amax 3, s = 7) become 0, +-128, +-256, +-384, all
+    exact in e4m3, so the product is the exact integer product.  A scale of
+    448 / amax would map 1 to 149.33 and round it."""
+    import portblas_amd as pb
+
+    torch = torch_cuda
+    g = torch.Generator(device="cuda").manual_seed(91)
+    A = torch.randint(-3, 4, (m * k,), generator=g, device="cuda").float()
+    B = torch.randint(-3, 4, (k * n,), generator=g, device="cuda").float()
+    A[0], B[0] = 3.0, -3.0
+    C = torch.zeros(m * n, device="cuda")
+    pb._gemm_ex(handle, pb.gemm_precision_t.fp8_e4m3, "n", "n", m, n, k, 1.0, A, m, B, k, 0.0, C, m)
+    torch.cuda.synchronize()
+    ref = cm(A, m, k, m).double() @ cm(B, k, n, k).double()
+    assert torch.equal(cm(C, m, n, m).double(), ref)
+
+
 def test_fp8_guessed_scales_match_fresh_handle(torch_cuda):
     """pb_gemm_ex(FP8) packs with the power-of-two scales of the handle's
     previous FP8 call and packs an operand again when its amax gives another
