@@ -374,6 +374,9 @@ __device__ __forceinline__ void zero_acc(Traits<float>::AccV (&acc)[2][2]) {
 // D problems after it, 8 DMAs (and, but for the last, 4 stores): the wait for
 // them is vmcnt(12 D) (vector memory operations complete in order), vmcnt(8 D)
 // in the first D steps (fewer stores behind them), vmcnt(0) in the last D.
+// The counts assume no other vector memory operations in the loop: no
+// register spills (the kernel needs about 150 of the 512 registers a lone
+// wave per SIMD may use).
 template <bool KCA, bool KCB, int NB>
 __global__ __launch_bounds__(256) void small_stream_kernel(GemmProblem<float> p) {
   constexpr int kElems = 2 * kTile * kTile;
